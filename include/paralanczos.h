@@ -1,0 +1,114 @@
+/*
+ * paralanczos.h - device-pointer C ABI of the MI355X batched-submatrix
+ * Lanczos eigenvalue library (libparalanczos.so).
+ *
+ * All array arguments are device pointers allocated by the caller (any HIP
+ * allocator); `stream` is a hipStream_t (NULL = default stream). Every call
+ * is asynchronous on `stream` unless stated otherwise; argument errors are
+ * reported synchronously through the returned pl_status and pl_last_error().
+ *
+ * Reference interfaces these replace (btuomanen/ParaLanczos):
+ *   pl_solve          - POS_SYM_SUBMATRIX_KER / POS_SYM_SUBMATRIX_KER2
+ *                       (ParaLanczos.h:32-34, launched at ParaLanczos.cu:1754,
+ *                       :1852 and by Matlab/gpu_multi_subsvd.m:54-60), reduced
+ *                       to lambda_min, lambda_max and the RIP deviation.
+ *   pl_solve_alleig   - the same kernels' full outputs Eigenvalues[s*k + j]
+ *                       ascending, DetMat[s], Eigerror[s]
+ *                       (ParaLanczos.cu:1465-1478).
+ *   pl_gen_*          - no reference counterpart: seeded index sets and
+ *                       synthetic sensing matrices (the reference takes a
+ *                       caller-supplied subIndex, ParaLanczos.cu:1811).
+ */
+#ifndef PARALANCZOS_AMD_H
+#define PARALANCZOS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  PL_OK = 0,
+  PL_ERR_ARG = 1,         /* invalid argument (see pl_last_error) */
+  PL_ERR_HIP = 2,         /* a HIP runtime call failed */
+  PL_ERR_NOMEM = 3,       /* device allocation failed */
+  PL_ERR_UNSUPPORTED = 4  /* valid request this build does not support */
+} pl_status;
+
+/* per-subset status bits written by pl_solve / pl_solve_alleig */
+#define PL_SUB_BREAKDOWN 1u /* Lanczos restarted (invariant subspace found) */
+#define PL_SUB_NONFINITE 2u /* non-finite entry in G_S or in T            */
+#define PL_SUB_NOCONV 4u    /* bisection hit its iteration cap            */
+#define PL_SUB_BADINDEX 8u  /* an index outside [0, n)                    */
+#define PL_SUB_BADK 16u     /* subset size outside the supported range    */
+
+/* solve forms */
+#define PL_FORM_TUNED 0 /* lane group = smallest of 8/16/32/64 >= k, tuned LDS pad */
+#define PL_FORM_PLAIN 1 /* one 64-lane wavefront per subset, one launch per k      */
+
+const char *pl_version(void);
+const char *pl_status_string(pl_status st);
+/* message of the last failing call on this thread ("" if none) */
+const char *pl_last_error(void);
+
+/* select the device for subsequent calls on this thread */
+pl_status pl_init(int device);
+
+/* largest subset size the solve kernels accept */
+int pl_max_k(void);
+
+/*
+ * lambda_min, lambda_max and delta = max(lambda_max - 1, 1 - lambda_min) of
+ * the principal submatrices G[S_s, S_s], s = 0..nsub-1, of the symmetric
+ * n x n FP32 matrix G (row-major, leading dimension ld >= n; only the upper
+ * triangle G[min(i,j)][max(i,j)] is read).
+ *   offsets == NULL: S_s = idx[s*k .. s*k+k-1] (all subsets of size k);
+ *   offsets != NULL: S_s = idx[offsets[s] .. offsets[s+1]-1] (int64 offsets,
+ *                    device memory) and k is the largest subset size.
+ * Outputs: lmin, lmax, delta (FP32) and status (PL_SUB_* bits) per subset.
+ * Subsets with status BADINDEX/BADK/NONFINITE get NaN eigenvalues.
+ */
+pl_status pl_solve(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                   const int64_t *offsets, int k, int64_t nsub, float *lmin, float *lmax,
+                   float *delta, uint32_t *status, int form, void *stream);
+
+/* same, with an explicit variant: lanes per subset (8/16/32/64, >= k) and LDS
+   row padding (0, 1 or 4 floats) - used by the autotuner */
+pl_status pl_solve_variant(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                           const int64_t *offsets, int k, int64_t nsub, float *lmin,
+                           float *lmax, float *delta, uint32_t *status, int lanes, int pad,
+                           void *stream);
+
+/* all k eigenvalues ascending eig[s*k + j] (FP64), det[s] = prod_j eig, and
+   err[s] = per-subset status, for nsub subsets of size k (uniform) */
+pl_status pl_solve_alleig(const float *G, int64_t n, int64_t ld, const int32_t *idx, int k,
+                          int64_t nsub, double *eig, double *det, int32_t *err, void *stream);
+
+/* gather stage only: out[s*k*k + r*k + c] = G_S[r][c] (mode 0), or
+   out[s] = sum of G_S (mode 1); uniform k */
+pl_status pl_gather(const float *G, int64_t n, int64_t ld, const int32_t *idx, int k,
+                    int64_t nsub, float *out, int mode, int lanes, int pad, void *stream);
+
+/* seeded index sets: Floyd sample of k distinct indices of [0,n) per subset
+   s0+s, Philox4x32-10 stream (seed, tag 1, s0+s), kept in draw order */
+pl_status pl_gen_subsets(uint64_t seed, int64_t s0, int64_t nsub, int k, int n, int32_t *idx,
+                         void *stream);
+/* same for variable sizes: subset s occupies idx[offsets[s] .. offsets[s+1]) */
+pl_status pl_gen_subsets_csr(uint64_t seed, int64_t s0, int64_t nsub, const int64_t *offsets,
+                             int n, int32_t *idx, void *stream);
+/* ks[s] = klist[u], u uniform in [0,nk) from stream (seed, tag 2, s0+s);
+   klist is a HOST array of at most 32 sizes */
+pl_status pl_gen_ks(uint64_t seed, int64_t s0, int64_t nsub, const int32_t *klist, int nk,
+                    int32_t *ks, void *stream);
+
+/* synthetic sensing matrix, column-major m x n (phi[j*m + r]):
+   kind 0: N(0,1/m) entries, columns normalised to unit L2 norm;
+   kind 1: +-1/sqrt(m) equiprobable;
+   kind 2: m seeded rows of the orthonormal n-point DCT-II, scaled sqrt(n/m) */
+pl_status pl_gen_phi(int kind, uint64_t seed, int m, int n, float *phi, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
