@@ -336,6 +336,7 @@ int legacy_solve(const double *mat, int matsize, const int *subIndex, int L, int
 void print_eigs(const double *eig, int count, const char *word) {
   if (quiet()) return;
   for (int i = 0; i < count; ++i) printf("%d %s: %g \n", i, word, eig[i]);
+  fflush(stdout);
 }
 
 }  // namespace
@@ -355,6 +356,7 @@ void print_tridiagonal(double *alpha, double *beta, int n) {
     }
     printf(r != n - 1 ? "; \n" : "] \n");
   }
+  fflush(stdout);
 }
 
 void calculate_eigs(double *mat, int n, double *Eigenvalues, double *Det) {
@@ -388,7 +390,10 @@ void calculate_subeigs2(double *mat, int n, int matsize, double *Eigenvalues, do
     return;
   }
   std::lock_guard<std::mutex> lock(g_legacy_mu);
-  if (!quiet()) printf("welcome to calculate subeigs2...\n");
+  if (!quiet()) {
+    printf("welcome to calculate subeigs2...\n");
+    fflush(stdout);
+  }
   if (legacy_solve(mat, matsize, subIndex, n, subn, Eigenvalues, n, Det, 1, nullptr, 0) != 0)
     return;
   print_eigs(Eigenvalues, n, "eigenvalue");
@@ -405,7 +410,10 @@ void calculate_subeigs3(double *mat, int subIndexLen, int matsize, double *Eigen
     return;
   }
   std::lock_guard<std::mutex> lock(g_legacy_mu);
-  if (!quiet()) printf("welcome to calculate subeigs2...\n");
+  if (!quiet()) {
+    printf("welcome to calculate subeigs2...\n");
+    fflush(stdout);
+  }
   if (legacy_solve(mat, matsize, subIndex, subIndexLen, subn, Eigenvalues, subIndexLen, Det,
                    subIndexLen / subn, nullptr, 0) != 0)
     return;
