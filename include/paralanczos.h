@@ -22,6 +22,7 @@
 #ifndef PARALANCZOS_AMD_H
 #define PARALANCZOS_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -72,6 +73,19 @@ int pl_max_k(void);
 pl_status pl_solve(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                    const int64_t *offsets, int k, int64_t nsub, float *lmin, float *lmax,
                    float *delta, uint32_t *status, int form, void *stream);
+
+/* Bytes of device workspace pl_solve_ws needs for nsub subsets of size <= k
+   (the tridiagonal matrices and their bounds between the two stages). */
+size_t pl_solve_workspace_size(int64_t nsub, int k);
+
+/* pl_solve with a caller-provided device workspace of workspace_bytes >=
+   pl_solve_workspace_size(nsub, k); with workspace == NULL it allocates one,
+   stream-ordered, as pl_solve does. Calls that share a workspace on
+   different streams race. */
+pl_status pl_solve_ws(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                      const int64_t *offsets, int k, int64_t nsub, float *lmin, float *lmax,
+                      float *delta, uint32_t *status, int form, void *workspace,
+                      size_t workspace_bytes, void *stream);
 
 /* same, with an explicit variant: lanes per subset (8/16/32/64, >= k) and LDS
    row padding (0, 1 or 4 floats) - used by the autotuner */

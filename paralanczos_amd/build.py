@@ -35,6 +35,9 @@ def flags() -> list[str]:
         "-fPIC",
         "-Wall",
         "-Wno-unused-function",
+        # every FMA in the kernels is explicit; implicit contraction would break
+        # the TwoSum error terms and make partner lanes compute different bits
+        "-ffp-contract=off",
         # no packed-FP32 instructions anywhere in the device code
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-I", os.path.join(HERE, "..", "include"),

@@ -62,14 +62,45 @@ pl_status check_solve_args(const float *G, int64_t n, int64_t ld, const int32_t 
   return PL_OK;
 }
 
+// stream-ordered workspace: the caller's, or one allocated for this call
+struct Workspace {
+  void *p = nullptr;
+  bool owned = false;
+  hipStream_t stream = nullptr;
+  pl_status acquire(void *given, size_t given_bytes, size_t need, hipStream_t s) {
+    stream = s;
+    if (given) {
+      if (given_bytes < need)
+        return fail(PL_ERR_ARG, "workspace of %zu bytes < %zu needed", given_bytes, need);
+      p = given;
+      return PL_OK;
+    }
+    hipError_t e = hipMallocAsync(&p, need ? need : 16, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(workspace)");
+    owned = true;
+    return PL_OK;
+  }
+  ~Workspace() {
+    if (owned) (void)hipFreeAsync(p, stream);
+  }
+};
+
 pl_status run_solve(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                     const int64_t *offsets, int k, int64_t nsub, float *lmin, float *lmax,
-                    float *delta, uint32_t *status, int group, int pad, void *stream) {
+                    float *delta, uint32_t *status, int group, int pad, void *ws,
+                    size_t ws_bytes, void *stream) {
   if (!lmin || !lmax || !delta || !status)
     return fail(PL_ERR_ARG, "output pointers must be non-null");
   if (group < k) return fail(PL_ERR_ARG, "lanes per subset %d < k = %d", group, k);
   if (!pl::solve_variant_exists(group, pad))
     return fail(PL_ERR_ARG, "no variant with %d lanes and pad %d", group, pad);
+  if (nsub == 0) return PL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Workspace w;
+  pl_status st = w.acquire(ws, ws_bytes, pl::workspace_bytes(nsub, k), s);
+  if (st != PL_OK) return st;
+  float2 *tri = (float2 *)w.p;
+  double4 *bounds = (double4 *)((char *)w.p + pl::tridiag_bytes(nsub, k));
   pl::SolveArgs a{};
   a.G = G;
   a.n = n;
@@ -78,12 +109,38 @@ pl_status run_solve(const float *G, int64_t n, int64_t ld, const int32_t *idx,
   a.offsets = offsets;
   a.kuni = offsets ? 0 : k;
   a.nslots = nsub;
-  a.lmin = lmin;
-  a.lmax = lmax;
-  a.delta = delta;
   a.status = status;
-  hipError_t e = pl::launch_solve(group, pad, pl::MODE_EXTREME, a, (hipStream_t)stream);
+  a.tri = tri;
+  a.tstride = k;
+  a.bounds = bounds;
+  hipError_t e = pl::launch_solve(group, pad, pl::MODE_TRIDIAG, a, s);
   if (e != hipSuccess) return hip_fail(e, "solve launch");
+  pl::BisectArgs b{};
+  b.tri = tri;
+  b.tstride = k;
+  b.bounds = bounds;
+  b.offsets = offsets;
+  b.kuni = offsets ? 0 : k;
+  b.nsub = nsub;
+  b.status = status;
+  b.lmin = lmin;
+  b.lmax = lmax;
+  b.delta = delta;
+  e = pl::launch_bisect(0, b, s);
+  if (e != hipSuccess) return hip_fail(e, "bisection launch");
+  return PL_OK;
+}
+
+pl_status pick_form(int form, int k, int &group, int &pad) {
+  if (form == PL_FORM_PLAIN) {
+    group = 64;
+    pad = 4;
+  } else if (form == PL_FORM_TUNED) {
+    group = group_for(k);
+    pad = tuned_pad(group);
+  } else {
+    return fail(PL_ERR_ARG, "unknown form %d", form);
+  }
   return PL_OK;
 }
 
@@ -114,22 +171,29 @@ pl_status pl_init(int device) {
 
 int pl_max_k(void) { return kMaxK; }
 
+size_t pl_solve_workspace_size(int64_t nsub, int k) {
+  if (nsub < 0 || k < 1) return 0;
+  return pl::workspace_bytes(nsub, k);
+}
+
 pl_status pl_solve(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                    const int64_t *offsets, int k, int64_t nsub, float *lmin, float *lmax,
                    float *delta, uint32_t *status, int form, void *stream) {
+  return pl_solve_ws(G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, form, nullptr, 0,
+                     stream);
+}
+
+pl_status pl_solve_ws(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                      const int64_t *offsets, int k, int64_t nsub, float *lmin, float *lmax,
+                      float *delta, uint32_t *status, int form, void *workspace,
+                      size_t workspace_bytes, void *stream) {
   pl_status st = check_solve_args(G, n, ld, idx, k, nsub);
   if (st != PL_OK) return st;
   int group, pad;
-  if (form == PL_FORM_PLAIN) {
-    group = 64;
-    pad = 4;
-  } else if (form == PL_FORM_TUNED) {
-    group = group_for(k);
-    pad = tuned_pad(group);
-  } else {
-    return fail(PL_ERR_ARG, "unknown form %d", form);
-  }
-  return run_solve(G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, group, pad, stream);
+  st = pick_form(form, k, group, pad);
+  if (st != PL_OK) return st;
+  return run_solve(G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, group, pad,
+                   workspace, workspace_bytes, stream);
 }
 
 pl_status pl_solve_variant(const float *G, int64_t n, int64_t ld, const int32_t *idx,
@@ -138,7 +202,8 @@ pl_status pl_solve_variant(const float *G, int64_t n, int64_t ld, const int32_t 
                            void *stream) {
   pl_status st = check_solve_args(G, n, ld, idx, k, nsub);
   if (st != PL_OK) return st;
-  return run_solve(G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, lanes, pad, stream);
+  return run_solve(G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, lanes, pad,
+                   nullptr, 0, stream);
 }
 
 pl_status pl_solve_alleig(const float *G, int64_t n, int64_t ld, const int32_t *idx, int k,
@@ -146,6 +211,13 @@ pl_status pl_solve_alleig(const float *G, int64_t n, int64_t ld, const int32_t *
   pl_status st = check_solve_args(G, n, ld, idx, k, nsub);
   if (st != PL_OK) return st;
   if (!eig || !det || !err) return fail(PL_ERR_ARG, "output pointers must be non-null");
+  if (nsub == 0) return PL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Workspace w;
+  st = w.acquire(nullptr, 0, pl::workspace_bytes(nsub, k), s);
+  if (st != PL_OK) return st;
+  float2 *tri = (float2 *)w.p;
+  double4 *bounds = (double4 *)((char *)w.p + pl::tridiag_bytes(nsub, k));
   const int group = group_for(k);
   pl::SolveArgs a{};
   a.G = G;
@@ -154,12 +226,23 @@ pl_status pl_solve_alleig(const float *G, int64_t n, int64_t ld, const int32_t *
   a.idx = idx;
   a.kuni = k;
   a.nslots = nsub;
-  a.eig = eig;
-  a.det = det;
-  a.err = err;
-  hipError_t e =
-      pl::launch_solve(group, tuned_pad(group), pl::MODE_ALLEIG, a, (hipStream_t)stream);
-  if (e != hipSuccess) return hip_fail(e, "all-eigenvalue launch");
+  a.status = (uint32_t *)err;
+  a.tri = tri;
+  a.tstride = k;
+  a.bounds = bounds;
+  hipError_t e = pl::launch_solve(group, tuned_pad(group), pl::MODE_TRIDIAG, a, s);
+  if (e != hipSuccess) return hip_fail(e, "all-eigenvalue solve launch");
+  pl::BisectArgs b{};
+  b.tri = tri;
+  b.tstride = k;
+  b.bounds = bounds;
+  b.kuni = k;
+  b.nsub = nsub;
+  b.status = (uint32_t *)err;
+  b.eig = eig;
+  b.det = det;
+  e = pl::launch_bisect(1, b, s);
+  if (e != hipSuccess) return hip_fail(e, "all-eigenvalue bisection launch");
   return PL_OK;
 }
 
@@ -243,7 +326,7 @@ pl_status pl_gen_phi(int kind, uint64_t seed, int m, int n, float *phi, void *st
     if (e == hipSuccess) e = pl::launch_gen_phi_dct(drows, m, n, phi, s);
     hipError_t e2 = hipStreamSynchronize(s);
     if (e == hipSuccess) e = e2;
-    hipFree(drows);
+    (void)hipFree(drows);
   } else {
     return fail(PL_ERR_ARG, "unknown phi kind %d", kind);
   }

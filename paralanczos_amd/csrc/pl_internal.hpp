@@ -8,10 +8,9 @@ namespace pl {
 
 // what a solve launch writes per subset
 enum Mode : int {
-  MODE_EXTREME = 0,   // lambda_min, lambda_max, delta, status (FP32 + u32)
-  MODE_ALLEIG = 1,    // all k eigenvalues ascending (FP64), det, err
-  MODE_DUMP = 2,      // the gathered k x k submatrix (FP32), uniform k only
-  MODE_CHECKSUM = 3,  // gather stage only: sum of G_S entries (FP32)
+  MODE_TRIDIAG = 0,   // stages (a)+(b): T (FP32 alpha, beta), its bounds, status
+  MODE_DUMP = 1,      // stage (a) only: the gathered k x k submatrix (FP32), uniform k
+  MODE_CHECKSUM = 2,  // stage (a) only: sum of the G_S entries (FP32)
 };
 
 struct SolveArgs {
@@ -22,16 +21,38 @@ struct SolveArgs {
   int kuni;                // subset size when offsets == nullptr
   int64_t nslots;          // work items in this launch
   const int32_t *order;    // slot -> subset id, or nullptr (identity)
-  float *lmin, *lmax, *delta;
-  uint32_t *status;
-  double *eig, *det;
-  int32_t *err;
+  uint32_t *status;        // per-subset status word
+  float2 *tri;             // T of subset s at tri + s*tstride, (alpha_i, beta_i)
+  int tstride;
+  double4 *bounds;         // (Gershgorin lo, hi, min alpha, max alpha) per subset
   float *gs_out;           // MODE_DUMP / MODE_CHECKSUM output
 };
+
+struct BisectArgs {
+  const float2 *tri;
+  int tstride;
+  const double4 *bounds;
+  const int64_t *offsets;  // as in SolveArgs
+  int kuni;
+  int64_t nsub;
+  uint32_t *status;        // read (fatal bits) and updated (NOCONV)
+  float *lmin, *lmax, *delta;
+  double *eig, *det;
+};
+
+// workspace of a solve: T entries, then (256-byte aligned) per-subset bounds
+inline size_t tridiag_bytes(int64_t nsub, int kstride) {
+  return ((size_t)nsub * (size_t)kstride * sizeof(float2) + 255) & ~(size_t)255;
+}
+inline size_t workspace_bytes(int64_t nsub, int kstride) {
+  return tridiag_bytes(nsub, kstride) + (size_t)nsub * sizeof(double4);
+}
 
 // lanes per subset (8, 16, 32, 64) x LDS row padding (0, 1, 4) x mode
 hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStream_t s);
 bool solve_variant_exists(int group, int pad);
+// mode 0: lambda_min/lambda_max/delta; mode 1: every eigenvalue + det (uniform k)
+hipError_t launch_bisect(int mode, const BisectArgs &a, hipStream_t s);
 
 hipError_t launch_gen_subsets(uint64_t seed, int64_t s0, int64_t nsub, const int64_t *offsets,
                               int kuni, int n, int32_t *idx, hipStream_t s);

@@ -6,21 +6,29 @@
 // the gathered submatrix G_S, entry i of every Lanczos vector, row i of the
 // Lanczos basis V (vrow) and the whole Lanczos vector v_i (vcol). Rows and
 // columns k..GROUP-1 are zero padding; they stay exactly zero through the
-// recurrence, so a subset gives the same arithmetic in any GROUP >= k.
+// recurrence, and every sum runs in the same order for any GROUP >= k, so a
+// subset gives bit-identical Lanczos arithmetic in any lane-group size.
 //
 // Stages per subset (reference: POS_SYM_SUBMATRIX_KER2, ParaLanczos.cu:1157-1480):
 //  (a) gather (ParaLanczos.cu:1237-1242 / :471-490): the upper triangle of G_S
 //      in global index order, G[min(i,j)][max(i,j)], is loaded one subset row
 //      per instruction (lanes = columns), mirrored into a padded LDS tile and
 //      each lane then reads its row into VGPRs.
-//  (b) k Lanczos steps (ParaLanczos.cu:1285-1374) in FP32: LDS all-gather of
-//      v and matvec from the VGPR row; alpha = v^T G v by a compensated
-//      (TwoProduct/TwoSum) group reduction; three-term update; one full
-//      classical Gram-Schmidt pass against all previous vectors; beta = ||w||
-//      compensated; deterministic restart on breakdown.
-//  (c) FP64 Sturm-count multisection on T for lambda_min and lambda_max
+//  (b) k Lanczos steps (ParaLanczos.cu:1285-1374) in FP32: matvec from the
+//      VGPR row; alpha = v^T G v by a compensated (TwoProduct/TwoSum) group
+//      reduction; three-term update; one full classical Gram-Schmidt pass
+//      against all previous vectors; beta = ||w|| compensated; deterministic
+//      restart on breakdown. The all-to-all broadcasts of v, w and the
+//      Gram-Schmidt coefficients use DPP row broadcasts fused into v_fmac for
+//      groups of 16/32/64 lanes (v_permlane16/32_swap bring the other rows'
+//      values into each row), and LDS for groups of 8.
+//  (c) FP64 Sturm-sequence multisection on T for lambda_min and lambda_max
 //      (replaces the serial TQLRAT of ParaLanczos.cu:32-189, :1454-1462), or
 //      per-lane bisection for every eigenvalue (legacy all-eigenvalue mode).
+//      The count uses the characteristic-polynomial recurrence (Barth, Martin,
+//      Wilkinson, Numer. Math. 9, 1967; the reference's unused BM_POLY,
+//      ParaLanczos.cu:504-552, without its off-by-one defects), which needs no
+//      division, rescaled by a power of two every 8 steps.
 // No kernel uses a block-level barrier: all cross-lane traffic stays inside
 // one wavefront, so groups finish independently.
 #include <float.h>
@@ -33,10 +41,9 @@
 namespace pl {
 
 constexpr int BLOCK = 256;
-constexpr int MS_MAXIT = 96;      // multisection iteration cap
-constexpr int MS_TOLBITS = 34;    // stop at width <= 2^-34 * max(|lo|,|hi|)
-constexpr int BIS_MAXIT = 160;    // all-eigenvalue bisection cap
-constexpr int BIS_TOLBITS = 44;
+constexpr int BIS_MAXIT = 64;        // iteration cap (the floor below bounds it by ~54)
+constexpr int BIS_TOLBITS_EXT = 27;  // extremes (FP32 outputs): width <= 2^-27 max(|lo|,|hi|)
+constexpr int BIS_TOLBITS_ALL = 44;  // all eigenvalues (FP64 legacy outputs)
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -44,7 +51,7 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// ---- group reductions (xor butterflies: every lane ends with identical bits) ----
+// ---- generic group reductions through ds_bpermute (setup paths, GROUP 8) ----
 template <int W>
 __device__ __forceinline__ uint32_t g_or(uint32_t x) {
 #pragma unroll
@@ -76,30 +83,6 @@ __device__ __forceinline__ float g_sum(float x) {
   return x;
 }
 
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
-  s = a + b;
-  const float bb = s - a;
-  e = (a - (s - bb)) + (b - bb);
-}
-
-// compensated dot product over the group (Ogita, Rump, Oishi, "Accurate sum
-// and dot product", SIAM J. Sci. Comput. 2005, Dot2 with a pairwise tree)
-template <int W>
-__device__ __forceinline__ float g_dot2(float x, float y) {
-  float p = x * y;
-  float e = __builtin_fmaf(x, y, -p);
-#pragma unroll
-  for (int m = 1; m < W; m <<= 1) {
-    const float po = __shfl_xor(p, m, W);
-    const float eo = __shfl_xor(e, m, W);
-    float s, t;
-    two_sum(p, po, s, t);
-    p = s;
-    e = (e + eo) + t;
-  }
-  return p + e;
-}
-
 // true when pred holds on every lane of the caller's group
 template <int W>
 __device__ __forceinline__ bool g_all(bool pred) {
@@ -113,8 +96,151 @@ __device__ __forceinline__ bool g_all(bool pred) {
   }
 }
 
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// ---- DPP / permlane building blocks (groups of 16, 32, 64 lanes) ----
+constexpr int DPP_QUAD_1032 = 0xB1;     // quad_perm [1,0,3,2]  (partner lane ^ 1)
+constexpr int DPP_QUAD_2301 = 0x4E;     // quad_perm [2,3,0,1]  (partner lane ^ 2)
+constexpr int DPP_ROW_MIRROR = 0x140;   // lane i <- 15 - i within a row of 16
+constexpr int DPP_ROW_HMIRROR = 0x141;  // lane i <- 7 - i within each half row
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL, 0xf, 0xf, true));
+}
+
+__device__ __forceinline__ float swap_lo16(float x, float &hi) {
+  // v_permlane16_swap with both operands x: result 0 holds row 2h's values and
+  // result 1 row 2h+1's values in every lane of the 32-lane half h
+  const auto r =
+      __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, x), __builtin_bit_cast(unsigned, x), false, false);
+  hi = __builtin_bit_cast(float, (unsigned)r[1]);
+  return __builtin_bit_cast(float, (unsigned)r[0]);
+}
+
+__device__ __forceinline__ float swap_lo32(float x, float &hi) {
+  // v_permlane32_swap with both operands x: result 0 holds lanes 0..31's
+  // values, result 1 lanes 32..63's, in both halves
+  const auto r =
+      __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, x), __builtin_bit_cast(unsigned, x), false, false);
+  hi = __builtin_bit_cast(float, (unsigned)r[1]);
+  return __builtin_bit_cast(float, (unsigned)r[0]);
+}
+
+// x of every lane of the group, split in rows of 16: part p holds, in every
+// lane, the values of row p of the group (lane 16p + n at row position n)
+template <int GROUP>
+__device__ __forceinline__ void row_parts(float x, float (&p)[GROUP / 16]) {
+  if constexpr (GROUP == 16) {
+    p[0] = x;
+  } else if constexpr (GROUP == 32) {
+    p[0] = swap_lo16(x, p[1]);
+  } else {
+    float hi32;
+    const float lo32 = swap_lo32(x, hi32);
+    p[0] = swap_lo16(lo32, p[1]);
+    p[2] = swap_lo16(hi32, p[3]);
+  }
+}
+
+// Fused row broadcast + FMA: a0 += sum_{n even} bcast_n(v) * g[n] and
+// a1 += sum_{n odd} bcast_n(v) * g[n] over one row of 16 lanes (v_fmac_f32
+// with DPP row_newbcast:n on its first source). The s_nop covers the DPP
+// read-after-VALU-write and VALU-exec-write hazards (at most 5 wait states).
+#define PL_BC(N, ACC, G)                                                               \
+  "v_fmac_f32_dpp %" #ACC ", %2, %" #G " row_newbcast:" #N " row_mask:0xf bank_mask:0xf\n\t"
+__device__ __forceinline__ void row_dot16(float &a0, float &a1, float v, const float *g) {
+  asm volatile("s_nop 4\n\t" PL_BC(0, 0, 3) PL_BC(1, 1, 4) PL_BC(2, 0, 5) PL_BC(3, 1, 6)
+                   PL_BC(4, 0, 7) PL_BC(5, 1, 8) PL_BC(6, 0, 9) PL_BC(7, 1, 10) PL_BC(8, 0, 11)
+                       PL_BC(9, 1, 12) PL_BC(10, 0, 13) PL_BC(11, 1, 14) PL_BC(12, 0, 15)
+                           PL_BC(13, 1, 16) PL_BC(14, 0, 17) PL_BC(15, 1, 18)
+               : "+v"(a0), "+v"(a1)
+               : "v"(v), "v"(g[0]), "v"(g[1]), "v"(g[2]), "v"(g[3]), "v"(g[4]), "v"(g[5]),
+                 "v"(g[6]), "v"(g[7]), "v"(g[8]), "v"(g[9]), "v"(g[10]), "v"(g[11]), "v"(g[12]),
+                 "v"(g[13]), "v"(g[14]), "v"(g[15]));
+}
+#undef PL_BC
+
+// sum_c x[c] * (value of lane c of the group) for a group of 16/32/64 lanes
+template <int GROUP>
+__device__ __forceinline__ float group_dot_dpp(float v, const float (&x)[GROUP]) {
+  float parts[GROUP / 16];
+  row_parts<GROUP>(v, parts);
+  float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+  for (int p = 0; p < GROUP / 16; ++p) row_dot16(a0, a1, parts[p], x + 16 * p);
+  return a0 + a1;
+}
+
+// Group butterfly over DPP/permlane partners; every lane ends with identical bits.
+template <int GROUP>
+__device__ __forceinline__ float g_sum_dpp(float x) {
+  x += dpp<DPP_QUAD_1032>(x);
+  x += dpp<DPP_QUAD_2301>(x);
+  x += dpp<DPP_ROW_HMIRROR>(x);
+  x += dpp<DPP_ROW_MIRROR>(x);
+  if constexpr (GROUP >= 32) {
+    float b;
+    const float a = swap_lo16(x, b);
+    x = a + b;
+  }
+  if constexpr (GROUP == 64) {
+    float b;
+    const float a = swap_lo32(x, b);
+    x = a + b;
+  }
+  return x;
+}
+
+// compensated dot product over the group (Ogita, Rump, Oishi, "Accurate sum
+// and dot product", SIAM J. Sci. Comput. 2005, Dot2 with a pairwise tree)
+__device__ __forceinline__ void comp_step(float &p, float &e, float po, float eo) {
+  float s, t;
+  two_sum(p, po, s, t);
+  p = s;
+  e = (e + eo) + t;
+}
+
+template <int GROUP>
+__device__ __forceinline__ float g_dot2(float x, float y) {
+  float p = x * y;
+  float e = __builtin_fmaf(x, y, -p);
+  if constexpr (GROUP < 16) {
+#pragma unroll
+    for (int m = 1; m < GROUP; m <<= 1) comp_step(p, e, __shfl_xor(p, m, GROUP), __shfl_xor(e, m, GROUP));
+  } else {
+    comp_step(p, e, dpp<DPP_QUAD_1032>(p), dpp<DPP_QUAD_1032>(e));
+    comp_step(p, e, dpp<DPP_QUAD_2301>(p), dpp<DPP_QUAD_2301>(e));
+    comp_step(p, e, dpp<DPP_ROW_HMIRROR>(p), dpp<DPP_ROW_HMIRROR>(e));
+    comp_step(p, e, dpp<DPP_ROW_MIRROR>(p), dpp<DPP_ROW_MIRROR>(e));
+    if constexpr (GROUP >= 32) {
+      float pb, eb;
+      const float pa = swap_lo16(p, pb);
+      const float ea = swap_lo16(e, eb);
+      p = pa;
+      e = ea;
+      comp_step(p, e, pb, eb);
+    }
+    if constexpr (GROUP == 64) {
+      float pb, eb;
+      const float pa = swap_lo32(p, pb);
+      const float ea = swap_lo32(e, eb);
+      p = pa;
+      e = ea;
+      comp_step(p, e, pb, eb);
+    }
+  }
+  return p + e;
+}
+
+// ---- LDS broadcast path (groups of 8) ----
 // sum_c x[c] * L[c] with L a 16-byte aligned LDS vector read by broadcast;
-// two accumulators split by index parity (fixed order for every GROUP)
+// two accumulators split by index parity (the order of row_dot16)
 template <int GROUP>
 __device__ __forceinline__ float bcast_dot(const float *L, const float (&x)[GROUP]) {
   float acc0 = 0.f, acc1 = 0.f;
@@ -129,12 +255,13 @@ __device__ __forceinline__ float bcast_dot(const float *L, const float (&x)[GROU
   return acc0 + acc1;
 }
 
-// one classical Gram-Schmidt pass of w (entry per lane) against the basis:
-// h_l = v_l . w computed by lane l from its vcol, then w -= sum_l V[i][l] h_l
+// Stage (b) for groups of 8 (LDS broadcasts): one classical Gram-Schmidt pass
+// of w against the basis, h_l = v_l . w by lane l from its vcol, then
+// w -= sum_l V[i][l] h_l with lane i's basis row in vrow
 template <int GROUP>
-__device__ __forceinline__ float cgs_pass(float w, const float (&vrow)[GROUP],
-                                          const float (&vcol)[GROUP], float *Lw, float *Lh,
-                                          int gl) {
+__device__ __forceinline__ float cgs_pass_lds(float w, const float (&vrow)[GROUP],
+                                              const float (&vcol)[GROUP], float *Lw, float *Lh,
+                                              int gl) {
   Lw[gl] = w;
   wave_sync();
   const float h = bcast_dot<GROUP>(Lw, vcol);
@@ -143,14 +270,32 @@ __device__ __forceinline__ float cgs_pass(float w, const float (&vrow)[GROUP],
   return w - bcast_dot<GROUP>(Lh, vrow);
 }
 
-// Stage (b): k Lanczos steps. T goes to Lt as interleaved (alpha_i, beta_i),
-// beta_0 = 0 and beta_i coupling rows i-1 and i.
+// index of the first lane of largest residual 1 - ||V[i][:]||^2 (restart rule)
 template <int GROUP>
-__device__ uint32_t lanczos_core(const float (&g)[GROUP], int k, int gl, float *R) {
+__device__ __forceinline__ int restart_lane(float res, int gl, int k) {
+  float bv = (gl < k) ? res : -1.f;
+  int bi = gl;
+#pragma unroll
+  for (int m = 1; m < GROUP; m <<= 1) {
+    const float ov = __shfl_xor(bv, m, GROUP);
+    const int oi = __shfl_xor(bi, m, GROUP);
+    if (ov > bv || (ov == bv && oi < bi)) {
+      bv = ov;
+      bi = oi;
+    }
+  }
+  return bi;
+}
+
+// Lanczos recurrence shared by both broadcast paths, ParaLanczos.cu:1285-1374
+// with the build's substitutions (ones/sqrt(k) start, full reorthogonalisation,
+// restart). Lane i ends with my_a = alpha_i and my_b = beta_i (beta_0 = 0).
+template <int GROUP>
+__device__ uint32_t lanczos_lds(const float (&g)[GROUP], int k, int gl, float *R, float &my_a,
+                                float &my_b) {
   float *Lv = R;
   float *Lw = R + GROUP;
   float *Lh = R + 2 * GROUP;
-  float *Lt = R + 3 * GROUP;
   float vrow[GROUP], vcol[GROUP];
 #pragma unroll
   for (int l = 0; l < GROUP; ++l) {
@@ -161,11 +306,11 @@ __device__ uint32_t lanczos_core(const float (&g)[GROUP], int k, int gl, float *
   vrow[0] = vi;
   float vp = 0.f, bj = 0.f, amax = 0.f, bmax = 0.f;
   uint32_t st = 0;
-  if (gl == 0) Lt[1] = 0.f;
+  my_a = 0.f;
+  my_b = 0.f;
   for (int j = 0; j < k; ++j) {
     Lv[gl] = vi;
     wave_sync();
-    // matvec w_i = sum_c G_S[i][c] v_c; lane j keeps v_j as its basis vector
     const bool mine = (gl == j);
     float acc0 = 0.f, acc1 = 0.f;
 #pragma unroll
@@ -182,17 +327,18 @@ __device__ uint32_t lanczos_core(const float (&g)[GROUP], int k, int gl, float *
     }
     float w = acc0 + acc1;
     const float a = g_dot2<GROUP>(vi, w);
+    my_a = mine ? a : my_a;
     amax = fmaxf(amax, fabsf(a));
     w = __builtin_fmaf(-a, vi, __builtin_fmaf(-bj, vp, w));
-    w = cgs_pass<GROUP>(w, vrow, vcol, Lw, Lh, gl);
-    if (gl == 0) Lt[2 * j] = a;
+    w = cgs_pass_lds<GROUP>(w, vrow, vcol, Lw, Lh, gl);
     if (j == k - 1) break;
     const float b = sqrtf(g_dot2<GROUP>(w, w));
     vp = vi;
+    float scale;
     if (b > TAU * (amax + bmax)) {
       bj = b;
       bmax = fmaxf(bmax, b);
-      vi = w * (1.0f / b);
+      scale = 1.0f / b;
     } else {
       // restart: e_i for the first i of largest residual 1 - ||V[i][:]||^2,
       // orthogonalised twice against the basis; beta_{j+1} = 0
@@ -201,130 +347,161 @@ __device__ uint32_t lanczos_core(const float (&g)[GROUP], int k, int gl, float *
       float res = 1.f;
 #pragma unroll
       for (int l = 0; l < GROUP; ++l) res = __builtin_fmaf(-vrow[l], vrow[l], res);
-      if (gl >= k) res = -1.f;
-      float bv = res;
-      int bi = gl;
-#pragma unroll
-      for (int m = 1; m < GROUP; m <<= 1) {
-        const float ov = __shfl_xor(bv, m, GROUP);
-        const int oi = __shfl_xor(bi, m, GROUP);
-        if (ov > bv || (ov == bv && oi < bi)) {
-          bv = ov;
-          bi = oi;
-        }
-      }
-      float r = (gl == bi) ? 1.f : 0.f;
-      r = cgs_pass<GROUP>(r, vrow, vcol, Lw, Lh, gl);
-      r = cgs_pass<GROUP>(r, vrow, vcol, Lw, Lh, gl);
-      const float nb = sqrtf(g_dot2<GROUP>(r, r));
-      vi = r * (1.0f / nb);
+      const int bi = restart_lane<GROUP>(res, gl, k);
+      w = (gl == bi) ? 1.f : 0.f;
+      w = cgs_pass_lds<GROUP>(w, vrow, vcol, Lw, Lh, gl);
+      w = cgs_pass_lds<GROUP>(w, vrow, vcol, Lw, Lh, gl);
+      scale = 1.0f / sqrtf(g_dot2<GROUP>(w, w));
     }
-    if (gl == 0) Lt[2 * (j + 1) + 1] = bj;
+    vi = w * scale;
 #pragma unroll
     for (int l = 1; l < GROUP; ++l) vrow[l] = (l == j + 1) ? vi : vrow[l];
+    my_b = (gl == j + 1) ? bj : my_b;
   }
+  return st;
+}
+
+// row gl of the basis (LDS, row stride STRIDE floats) into registers
+template <int GROUP, int STRIDE>
+__device__ __forceinline__ void load_basis_row(const float *V, int gl, float (&x)[GROUP]) {
+  if constexpr (STRIDE % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < GROUP / 4; ++q) {
+      const float4 t = reinterpret_cast<const float4 *>(V + gl * STRIDE)[q];
+      x[4 * q + 0] = t.x;
+      x[4 * q + 1] = t.y;
+      x[4 * q + 2] = t.z;
+      x[4 * q + 3] = t.w;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < GROUP; ++c) x[c] = V[gl * STRIDE + c];
+  }
+}
+
+// column gl of the basis: entries 0..GROUP-1 of Lanczos vector v_gl
+// (consecutive lanes read consecutive LDS words: no bank conflicts)
+template <int GROUP, int STRIDE>
+__device__ __forceinline__ void load_basis_col(const float *V, int gl, float (&x)[GROUP]) {
+#pragma unroll
+  for (int r = 0; r < GROUP; ++r) x[r] = V[r * STRIDE + gl];
+}
+
+// Gram-Schmidt pass for groups of 16/32/64 (DPP broadcasts) against the LDS
+// basis V (V[i*STRIDE + l] = entry i of v_l, zero for vectors not yet formed):
+// lane l forms h_l = v_l . w from column l, lane i subtracts sum_l V[i][l] h_l
+template <int GROUP, int STRIDE>
+__device__ __forceinline__ float cgs_pass_dpp(float w, const float *V, int gl) {
+  float x[GROUP];
+  load_basis_col<GROUP, STRIDE>(V, gl, x);
+  const float h = group_dot_dpp<GROUP>(w, x);
+  load_basis_row<GROUP, STRIDE>(V, gl, x);
+  return w - group_dot_dpp<GROUP>(h, x);
+}
+
+template <int GROUP, int STRIDE>
+__device__ uint32_t lanczos_dpp(const float (&g)[GROUP], int k, int gl, float *R, float &my_a,
+                                float &my_b) {
+  float *V = R;  // the gather tile is dead: reuse it as the basis
+  float vi = (gl < k) ? (1.0f / sqrtf((float)k)) : 0.f;
   wave_sync();
-  bool finite = true;
-  if (gl < k) finite = isfinite(Lt[2 * gl]) && isfinite(Lt[2 * gl + 1]);
+  if constexpr (STRIDE % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < GROUP / 4; ++q)
+      reinterpret_cast<float4 *>(V + gl * STRIDE)[q] = make_float4(q == 0 ? vi : 0.f, 0.f, 0.f, 0.f);
+  } else {
+#pragma unroll
+    for (int c = 0; c < GROUP; ++c) V[gl * STRIDE + c] = (c == 0) ? vi : 0.f;
+  }
+  float vp = 0.f, bj = 0.f, amax = 0.f, bmax = 0.f;
+  uint32_t st = 0;
+  my_a = 0.f;
+  my_b = 0.f;
+  for (int j = 0; j < k; ++j) {
+    // matvec w_i = sum_c G_S[i][c] v_c
+    float w = group_dot_dpp<GROUP>(vi, g);
+    const float a = g_dot2<GROUP>(vi, w);
+    my_a = (gl == j) ? a : my_a;
+    amax = fmaxf(amax, fabsf(a));
+    w = __builtin_fmaf(-a, vi, __builtin_fmaf(-bj, vp, w));
+    wave_sync();
+    w = cgs_pass_dpp<GROUP, STRIDE>(w, V, gl);
+    if (j == k - 1) break;
+    const float b = sqrtf(g_dot2<GROUP>(w, w));
+    vp = vi;
+    float scale;
+    if (b > TAU * (amax + bmax)) {
+      bj = b;
+      bmax = fmaxf(bmax, b);
+      scale = 1.0f / b;
+    } else {
+      // restart: e_i for the first i of largest residual 1 - ||V[i][:]||^2,
+      // orthogonalised twice against the basis; beta_{j+1} = 0
+      st |= ST_BREAKDOWN;
+      bj = 0.f;
+      float vr[GROUP];
+      load_basis_row<GROUP, STRIDE>(V, gl, vr);
+      float res = 1.f;
+#pragma unroll
+      for (int l = 0; l < GROUP; ++l) res = __builtin_fmaf(-vr[l], vr[l], res);
+      const int bi = restart_lane<GROUP>(res, gl, k);
+      w = (gl == bi) ? 1.f : 0.f;
+      w = cgs_pass_dpp<GROUP, STRIDE>(w, V, gl);
+      w = cgs_pass_dpp<GROUP, STRIDE>(w, V, gl);
+      scale = 1.0f / sqrtf(g_dot2<GROUP>(w, w));
+    }
+    vi = w * scale;
+    V[gl * STRIDE + j + 1] = vi;
+    wave_sync();
+    my_b = (gl == j + 1) ? bj : my_b;
+  }
+  return st;
+}
+
+template <int GROUP, int STRIDE>
+__device__ __forceinline__ uint32_t lanczos_core(const float (&g)[GROUP], int k, int gl, float *R,
+                                                 float &my_a, float &my_b) {
+  uint32_t st;
+  if constexpr (GROUP >= 16) st = lanczos_dpp<GROUP, STRIDE>(g, k, gl, R, my_a, my_b);
+  else st = lanczos_lds<GROUP>(g, k, gl, R, my_a, my_b);
+  const bool finite = (gl >= k) || (isfinite(my_a) && isfinite(my_b));
   if (!g_all<GROUP>(finite)) st |= ST_NONFINITE;
   return st;
 }
 
-// number of eigenvalues of T below sigma (LAPACK dstebz-style pivot guard)
-__device__ __forceinline__ int negcount(const float *Lt, int k, double sigma, double pivmin) {
-  double d = (double)Lt[0] - sigma;
-  if (fabs(d) < pivmin) d = -pivmin;
-  int c = d < 0.0;
-  for (int i = 1; i < k; ++i) {
-    const float2 ab = reinterpret_cast<const float2 *>(Lt)[i];
-    const double b = (double)ab.y;
-    d = ((double)ab.x - sigma) - (b * b) / d;
-    if (fabs(d) < pivmin) d = -pivmin;
-    c += d < 0.0;
-  }
-  return c;
-}
-
-// Gershgorin interval of T (widened as LAPACK dstebz does) and pivmin
+// Stage (b) epilogue: T to the workspace, tri[i] = (alpha_i, beta_i) in FP32
+// (beta_0 = 0), and the subset's bracket data in FP64: the Gershgorin interval
+// of T and the range of its diagonal
 template <int GROUP>
-__device__ __forceinline__ void t_bounds(const float *Lt, int k, int gl, double &lo, double &hi,
-                                         double &pivmin) {
-  double glo = INFINITY, ghi = -INFINITY, b2 = 0.0;
+__device__ __forceinline__ void store_tridiag(float2 *tri, double4 *bounds, int k, int gl,
+                                              float my_a, float my_b) {
+  const double al = my_a;
+  const double b = my_b;
+  const double bn = __shfl_down(b, 1, GROUP);  // beta_{i+1}
+  double glo = INFINITY, ghi = -INFINITY, amin = INFINITY, amax = -INFINITY;
   if (gl < k) {
-    const double al = Lt[2 * gl];
-    const double bl = (gl > 0) ? fabs((double)Lt[2 * gl + 1]) : 0.0;
-    const double br = (gl + 1 < k) ? fabs((double)Lt[2 * (gl + 1) + 1]) : 0.0;
-    glo = al - bl - br;
-    ghi = al + bl + br;
-    b2 = bl * bl;
+    tri[gl] = make_float2(my_a, gl > 0 ? my_b : 0.f);
+    const double r = (gl > 0 ? fabs(b) : 0.0) + (gl + 1 < k ? fabs(bn) : 0.0);
+    glo = al - r;
+    ghi = al + r;
+    amin = amax = al;
   }
   glo = g_min<GROUP>(glo);
   ghi = g_max<GROUP>(ghi);
-  b2 = g_max<GROUP>(b2);
-  pivmin = DBL_MIN * fmax(1.0, b2);
-  const double tnorm = fmax(fabs(glo), fabs(ghi));
-  const double fudge = 2.0 * DBL_EPSILON * tnorm * k + 4.0 * pivmin;
-  lo = glo - fudge;
-  hi = ghi + fudge;
+  amin = g_min<GROUP>(amin);
+  amax = g_max<GROUP>(amax);
+  if (gl == 0) *bounds = make_double4(glo, ghi, amin, amax);
 }
 
-// Stage (c), extreme eigenvalues: the group's lower half brackets lambda_min
-// (index 0), the upper half lambda_max (index k-1). Each iteration every lane
-// counts at one of HALF equispaced interior points; by monotonicity of the
-// count the new bracket follows from a ballot popcount.
+// waves per SIMD each lane-group size is register-budgeted for (VGPRs <= 512/n)
 template <int GROUP>
-__device__ __forceinline__ double extreme_eig(const float *Lt, int k, int gl, uint32_t &st) {
-  constexpr int HALF = GROUP / 2;
-  double lo, hi, pivmin;
-  t_bounds<GROUP>(Lt, k, gl, lo, hi, pivmin);
-  const int target = (gl < HALF) ? 0 : k - 1;
-  const int p = gl % HALF;
-  const int base = __lane_id() - p;
-  int it = 0;
-  for (; it < MS_MAXIT; ++it) {
-    const double tol = ldexp(fmax(fabs(lo), fabs(hi)), -MS_TOLBITS) + 4.0 * pivmin;
-    const bool done = (hi - lo) <= tol;
-    if (g_all<GROUP>(done)) break;
-    const double step = (hi - lo) / (double)(HALF + 1);
-    const double sigma = lo + step * (double)(p + 1);
-    const int c = negcount(Lt, k, sigma, pivmin);
-    const uint64_t m = __ballot(c <= target);
-    const int cnt = __popcll((m >> base) & ((1ull << HALF) - 1ull));
-    if (!done) {
-      const double nlo = (cnt > 0) ? lo + step * (double)cnt : lo;
-      const double nhi = (cnt < HALF) ? lo + step * (double)(cnt + 1) : hi;
-      lo = nlo;
-      hi = nhi;
-    }
-  }
-  if (it == MS_MAXIT) st |= ST_NOCONV;
-  return lo + 0.5 * (hi - lo);
-}
-
-// Stage (c), all eigenvalues: lane t bisects eigenvalue t (ascending order,
-// as TQLRAT returns them, ParaLanczos.cu:168-185)
-template <int GROUP>
-__device__ __forceinline__ double lane_eig(const float *Lt, int k, int gl, uint32_t &st) {
-  double lo, hi, pivmin;
-  t_bounds<GROUP>(Lt, k, gl, lo, hi, pivmin);
-  bool conv = true;
-  if (gl < k) {
-    int it = 0;
-    for (; it < BIS_MAXIT; ++it) {
-      const double tol = ldexp(fmax(fabs(lo), fabs(hi)), -BIS_TOLBITS) + 4.0 * pivmin;
-      if (hi - lo <= tol) break;
-      const double mid = lo + 0.5 * (hi - lo);
-      if (negcount(Lt, k, mid, pivmin) > gl) hi = mid;
-      else lo = mid;
-    }
-    conv = it < BIS_MAXIT;
-  }
-  if (!g_all<GROUP>(conv)) st |= ST_NOCONV;
-  return lo + 0.5 * (hi - lo);
+constexpr int waves_per_simd() {
+  return GROUP == 8 ? 8 : GROUP == 16 ? 7 : GROUP == 32 ? 4 : 2;
 }
 
 template <int GROUP, int PAD, int MODE>
-__global__ __launch_bounds__(BLOCK) void solve_kernel(SolveArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(waves_per_simd<GROUP>())))
+void solve_kernel(SolveArgs a) {
   constexpr int SPB = BLOCK / GROUP;
   constexpr int STRIDE = GROUP + PAD;
   constexpr int REGION = GROUP * STRIDE;
@@ -412,29 +589,14 @@ __global__ __launch_bounds__(BLOCK) void solve_kernel(SolveArgs a) {
     if (gl == 0) a.gs_out[s] = st ? NAN : acc;
     return;
   } else {
-    const bool bad = (st & (ST_BADK | ST_BADINDEX | ST_NONFINITE)) != 0u;
-    if (!bad) st |= lanczos_core<GROUP>(g, k, gl, R);
-    const float *Lt = R + 3 * GROUP;
-    if constexpr (MODE == MODE_EXTREME) {
-      double lam = NAN;
-      if (!(st & (ST_BADK | ST_BADINDEX | ST_NONFINITE))) lam = extreme_eig<GROUP>(Lt, k, gl, st);
-      const double lmax = __shfl(lam, GROUP / 2, GROUP);
-      if (gl == 0) {
-        a.lmin[s] = (float)lam;
-        a.lmax[s] = (float)lmax;
-        a.delta[s] = (float)fmax(lmax - 1.0, 1.0 - lam);
-        a.status[s] = st;
-      }
-    } else {  // MODE_ALLEIG
-      double lam = NAN;
-      if (!(st & (ST_BADK | ST_BADINDEX | ST_NONFINITE))) lam = lane_eig<GROUP>(Lt, k, gl, st);
-      const double det = g_prod<GROUP>(gl < k ? lam : 1.0);
-      if (gl < k) a.eig[off + gl] = lam;
-      if (gl == 0) {
-        a.det[s] = det;
-        a.err[s] = (int32_t)st;
-      }
+    constexpr uint32_t FATAL = ST_BADK | ST_BADINDEX | ST_NONFINITE;
+    if (!(st & FATAL)) {
+      float my_a, my_b;
+      st |= lanczos_core<GROUP, STRIDE>(g, k, gl, R, my_a, my_b);
+      if (!(st & FATAL))
+        store_tridiag<GROUP>(a.tri + s * (int64_t)a.tstride, a.bounds + s, k, gl, my_a, my_b);
     }
+    if (gl == 0) a.status[s] = st;
   }
 }
 
@@ -442,8 +604,8 @@ using SolveFn = void (*)(SolveArgs);
 
 template <int GROUP, int PAD>
 struct Variant {
-  static constexpr SolveFn fns[4] = {solve_kernel<GROUP, PAD, 0>, solve_kernel<GROUP, PAD, 1>,
-                                     solve_kernel<GROUP, PAD, 2>, solve_kernel<GROUP, PAD, 3>};
+  static constexpr SolveFn fns[3] = {solve_kernel<GROUP, PAD, 0>, solve_kernel<GROUP, PAD, 1>,
+                                     solve_kernel<GROUP, PAD, 2>};
 };
 
 static SolveFn pick(int group, int pad, int mode) {
@@ -460,7 +622,7 @@ static SolveFn pick(int group, int pad, int mode) {
 bool solve_variant_exists(int group, int pad) { return pick(group, pad, 0) != nullptr; }
 
 hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStream_t s) {
-  if (mode < 0 || mode > 3) return hipErrorInvalidValue;
+  if (mode < 0 || mode > 2) return hipErrorInvalidValue;
   SolveFn fn = pick(group, pad, mode);
   if (!fn) return hipErrorInvalidValue;
   if (a.nslots <= 0) return hipSuccess;
@@ -468,6 +630,241 @@ hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStr
   const int64_t blocks = (a.nslots + spb - 1) / spb;
   if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
   hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(BLOCK), 0, s, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Stage (c): FP64 Sturm-sequence bisection, one lane per eigenvalue sought.
+// A block of 64 lanes stages the T of its subsets into LDS once (row j of the
+// stage holds subset j's (alpha_i, beta_i), rows kstride+1 float2 apart so the
+// staging writes and the per-step reads hit distinct banks) and every lane
+// then bisects its own target from LDS.
+// ---------------------------------------------------------------------------
+constexpr int BIS_BLOCK = 64;
+
+// Number of eigenvalues of T below sigma: sign changes of the Sturm sequence
+// p_0 = 1, p_1 = alpha_0 - sigma, p_i = (alpha_{i-1} - sigma) p_{i-1} -
+// beta_{i-1}^2 p_{i-2} (Barth, Martin, Wilkinson, Numer. Math. 9, 1967; the
+// reference's unused BM_POLY, ParaLanczos.cu:504-552, without its off-by-one
+// defects), evaluated in FP64 from the FP32 entries (exact conversions). It
+// needs no division. An exactly zero p_i is replaced by -2^-60 p_{i-1}, so it
+// takes the sign opposite to p_{i-1} (Wilkinson's convention) and the
+// recurrence continues through a vanishing beta; the pair (p_{i-1}, p_i) is
+// rescaled by a power of two every 8 steps so that nothing overflows.
+__device__ __forceinline__ int sturm_count(const float2 *tri, int k, double sigma) {
+  double p0 = 1.0;
+  double p1 = (double)tri[0].x - sigma;
+  if (p1 == 0.0) p1 = -0x1p-60;
+  bool neg1 = p1 < 0.0;
+  int c = neg1;
+  float2 t = tri[k > 1 ? 1 : 0];
+  for (int i = 1; i < k; ++i) {
+    const float2 tn = tri[i + 1 < k ? i + 1 : i];  // next entry, read ahead of the math
+    const double b = (double)t.y;
+    double p2 = __builtin_fma((double)t.x - sigma, p1, -(b * b) * p0);
+    p2 = (p2 == 0.0) ? -p1 * 0x1p-60 : p2;
+    t = tn;
+    const bool neg2 = p2 < 0.0;
+    c += neg2 != neg1;
+    neg1 = neg2;
+    p0 = p1;
+    p1 = p2;
+    if ((i & 7) == 0) {
+      int e;
+      (void)frexp(fmax(fabs(p0), fabs(p1)), &e);
+      p0 = ldexp(p0, -e);
+      p1 = ldexp(p1, -e);
+    }
+  }
+  return c;
+}
+
+// eigenvalue `target` (0-based, ascending) of T by bisection of [lo, hi];
+// stops at width <= max(2^-tolbits max(|lo|,|hi|), 2^-52 tnorm)
+__device__ __forceinline__ double bisect(const float2 *tri, int k, int target, double lo,
+                                         double hi, double tnorm, int tolbits, bool &conv) {
+  const double floor_tol = ldexp(tnorm, -52) + 0x1p-1000;
+  int it = 0;
+  for (; it < BIS_MAXIT; ++it) {
+    const double tol = fmax(ldexp(fmax(fabs(lo), fabs(hi)), -tolbits), floor_tol);
+    if (hi - lo <= tol) break;
+    const double mid = lo + 0.5 * (hi - lo);
+    if (sturm_count(tri, k, mid) > target) hi = mid;
+    else lo = mid;
+  }
+  conv = it < BIS_MAXIT;
+  return lo + 0.5 * (hi - lo);
+}
+
+// Gershgorin bounds widened as LAPACK dstebz does
+__device__ __forceinline__ void widened(const double4 &b, int k, double &glo, double &ghi,
+                                        double &tnorm) {
+  tnorm = fmax(fabs(b.x), fabs(b.y));
+  const double fudge = 2.0 * DBL_EPSILON * tnorm * k + 0x1p-1000;
+  glo = b.x - fudge;
+  ghi = b.y + fudge;
+}
+
+__device__ __forceinline__ int subset_k(const BisectArgs &a, int64_t s) {
+  return a.offsets ? (int)(a.offsets[s + 1] - a.offsets[s]) : a.kuni;
+}
+
+// copy the T of subsets s0 .. s0+cnt-1 into the LDS stage (one row per subset)
+__device__ __forceinline__ void stage_tridiag(const BisectArgs &a, int64_t s0, int cnt,
+                                              float2 *stage) {
+  const int rs = a.tstride + 1;
+  for (int j = 0; j < cnt; ++j) {
+    const float2 *src = a.tri + (s0 + j) * (int64_t)a.tstride;
+    for (int i = threadIdx.x; i < a.tstride; i += BIS_BLOCK) stage[j * rs + i] = src[i];
+  }
+  __syncthreads();
+}
+
+// Sturm counts of the same T at two shifts, interleaved so that the two
+// independent recurrences share the LDS reads and fill each other's latency
+__device__ __forceinline__ void sturm_count2(const float2 *tri, int k, double sa, double sb,
+                                             int &ca, int &cb) {
+  double a0 = 1.0, b0 = 1.0;
+  double a1 = (double)tri[0].x - sa;
+  double b1 = (double)tri[0].x - sb;
+  if (a1 == 0.0) a1 = -0x1p-60;
+  if (b1 == 0.0) b1 = -0x1p-60;
+  bool an = a1 < 0.0, bn = b1 < 0.0;
+  ca = an;
+  cb = bn;
+  float2 t = tri[k > 1 ? 1 : 0];
+  for (int i = 1; i < k; ++i) {
+    const float2 tn = tri[i + 1 < k ? i + 1 : i];
+    const double al = (double)t.x;
+    const double bt = (double)t.y;
+    const double b2 = bt * bt;
+    double a2 = __builtin_fma(al - sa, a1, -b2 * a0);
+    double b2s = __builtin_fma(al - sb, b1, -b2 * b0);
+    a2 = (a2 == 0.0) ? -a1 * 0x1p-60 : a2;
+    b2s = (b2s == 0.0) ? -b1 * 0x1p-60 : b2s;
+    t = tn;
+    const bool an2 = a2 < 0.0, bn2 = b2s < 0.0;
+    ca += an2 != an;
+    cb += bn2 != bn;
+    an = an2;
+    bn = bn2;
+    a0 = a1;
+    a1 = a2;
+    b0 = b1;
+    b1 = b2s;
+    if ((i & 7) == 0) {
+      int ea, eb;
+      (void)frexp(fmax(fabs(a0), fabs(a1)), &ea);
+      (void)frexp(fmax(fabs(b0), fabs(b1)), &eb);
+      a0 = ldexp(a0, -ea);
+      a1 = ldexp(a1, -ea);
+      b0 = ldexp(b0, -eb);
+      b1 = ldexp(b1, -eb);
+    }
+  }
+}
+
+// lambda_min / lambda_max: lane j of a block bisects both extremes of subset j
+// at once; brackets [Gershgorin lo, min alpha] and [max alpha, Gershgorin hi]
+// hold since lambda_min(T) <= T_ii <= lambda_max(T)
+__global__ __launch_bounds__(BIS_BLOCK) void bisect_extreme_kernel(BisectArgs a) {
+  extern __shared__ float2 stage[];
+  const int64_t s0 = (int64_t)blockIdx.x * BIS_BLOCK;
+  const int cnt = (int)min<int64_t>(BIS_BLOCK, a.nsub - s0);
+  stage_tridiag(a, s0, cnt, stage);
+  const int j = threadIdx.x;
+  if (j >= cnt) return;
+  const int64_t s = s0 + j;
+  const int k = subset_k(a, s);
+  uint32_t st = a.status[s];
+  double lmin = NAN, lmax = NAN;
+  if (!(st & (ST_BADK | ST_BADINDEX | ST_NONFINITE))) {
+    const float2 *tri = stage + j * (a.tstride + 1);
+    const double4 bd = a.bounds[s];
+    double glo, ghi, tnorm;
+    widened(bd, k, glo, ghi, tnorm);
+    const double floor_tol = ldexp(tnorm, -52) + 0x1p-1000;
+    double lo0 = glo, hi0 = bd.z, lo1 = bd.w, hi1 = ghi;
+    int it = 0;
+    for (; it < BIS_MAXIT; ++it) {
+      const double tol0 = fmax(ldexp(fmax(fabs(lo0), fabs(hi0)), -BIS_TOLBITS_EXT), floor_tol);
+      const double tol1 = fmax(ldexp(fmax(fabs(lo1), fabs(hi1)), -BIS_TOLBITS_EXT), floor_tol);
+      const bool d0 = hi0 - lo0 <= tol0, d1 = hi1 - lo1 <= tol1;
+      if (d0 && d1) break;
+      const double m0 = lo0 + 0.5 * (hi0 - lo0), m1 = lo1 + 0.5 * (hi1 - lo1);
+      int c0, c1;
+      sturm_count2(tri, k, m0, m1, c0, c1);
+      if (!d0) {
+        if (c0 > 0) hi0 = m0;
+        else lo0 = m0;
+      }
+      if (!d1) {
+        if (c1 > k - 1) hi1 = m1;
+        else lo1 = m1;
+      }
+    }
+    if (it == BIS_MAXIT) st |= ST_NOCONV;
+    lmin = lo0 + 0.5 * (hi0 - lo0);
+    lmax = lo1 + 0.5 * (hi1 - lo1);
+  }
+  a.lmin[s] = (float)lmin;
+  a.lmax[s] = (float)lmax;
+  a.delta[s] = (float)fmax(lmax - 1.0, 1.0 - lmin);
+  a.status[s] = st;
+}
+
+// every eigenvalue (uniform k): lane t of the block's targets bisects
+// eigenvalue t % k of subset t / k
+__global__ __launch_bounds__(BIS_BLOCK) void bisect_all_kernel(BisectArgs a) {
+  extern __shared__ float2 stage[];
+  const int k = a.kuni;
+  const int64_t t0 = (int64_t)blockIdx.x * BIS_BLOCK;
+  const int64_t s0 = t0 / k;
+  const int64_t s_last = min<int64_t>((t0 + BIS_BLOCK - 1) / k, a.nsub - 1);
+  stage_tridiag(a, s0, (int)(s_last - s0 + 1), stage);
+  const int64_t gid = t0 + threadIdx.x;
+  const int64_t s = gid / k;
+  if (s >= a.nsub) return;
+  const int t = (int)(gid - s * k);
+  const uint32_t st = a.status[s];
+  double lam = NAN;
+  if (!(st & (ST_BADK | ST_BADINDEX | ST_NONFINITE))) {
+    const float2 *tri = stage + (s - s0) * (a.tstride + 1);
+    double glo, ghi, tnorm;
+    widened(a.bounds[s], k, glo, ghi, tnorm);
+    bool conv;
+    lam = bisect(tri, k, t, glo, ghi, tnorm, BIS_TOLBITS_ALL, conv);
+    if (!conv) atomicOr(a.status + s, ST_NOCONV);
+  }
+  a.eig[gid] = lam;
+}
+
+// det = product of the eigenvalues (after bisect_all_kernel)
+__global__ __launch_bounds__(256) void det_kernel(BisectArgs a) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.nsub) return;
+  double d = 1.0;
+  for (int t = 0; t < a.kuni; ++t) d *= a.eig[s * a.kuni + t];
+  a.det[s] = d;
+}
+
+hipError_t launch_bisect(int mode, const BisectArgs &a, hipStream_t s) {
+  if (a.nsub <= 0) return hipSuccess;
+  if (mode == 0) {
+    const int64_t blocks = (a.nsub + BIS_BLOCK - 1) / BIS_BLOCK;
+    const size_t lds = (size_t)BIS_BLOCK * (a.tstride + 1) * sizeof(float2);
+    hipLaunchKernelGGL(bisect_extreme_kernel, dim3((unsigned)blocks), dim3(BIS_BLOCK), lds, s, a);
+  } else {
+    const int64_t threads = a.nsub * a.kuni;
+    const int64_t blocks = (threads + BIS_BLOCK - 1) / BIS_BLOCK;
+    // a block's 64 targets span at most 64/k + 2 subsets
+    const int rows = BIS_BLOCK / a.kuni + 2;
+    const size_t lds = (size_t)rows * (a.tstride + 1) * sizeof(float2);
+    hipLaunchKernelGGL(bisect_all_kernel, dim3((unsigned)blocks), dim3(BIS_BLOCK), lds, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(det_kernel, dim3((unsigned)((a.nsub + 255) / 256)), dim3(256), 0, s, a);
+  }
   return hipGetLastError();
 }
 
