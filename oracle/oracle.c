@@ -244,7 +244,9 @@ static void cgs_pass(const double *V, int nv, int k, double *w, double *h) {
    the three-term subtraction; beta_k discarded as in :1377-1378) with the
    build's substitutions: start vector ones/sqrt(k) instead of curand normals
    (:1196-1233), one full classical Gram-Schmidt pass against all previous
-   Lanczos vectors after the three-term step, and a deterministic restart when
+   Lanczos vectors after the three-term step (a second one when the first
+   removed more than half of the vector's norm: the DGKS criterion of Daniel,
+   Gragg, Kaufman, Stewart, Math. Comp. 1976), and a deterministic restart when
    beta_{j+1} <= OR_TAU * (max|alpha| + max beta): the new vector is e_i for
    the first i of largest residual 1 - sum_l V[i][l]^2, orthogonalised twice,
    and beta_{j+1} := 0. */
@@ -266,9 +268,15 @@ uint32_t or_lanczos(const double *A, int k, double *alpha, double *beta) {
     alpha[j] = a;
     if (fabs(a) > amax) amax = fabs(a);
     for (int i = 0; i < k; ++i) w[i] -= a * v[i] + beta[j] * vp[i];
+    const double pre = dotd(w, w, k);
     cgs_pass(V, j + 1, k, w, h);
     if (j == k - 1) break;
-    double b = sqrt(dotd(w, w, k));
+    double b2 = dotd(w, w, k);
+    if (b2 < 0.5 * pre) {  /* DGKS criterion: orthogonalise a second time */
+      cgs_pass(V, j + 1, k, w, h);
+      b2 = dotd(w, w, k);
+    }
+    double b = sqrt(b2);
     memcpy(vp, v, sizeof(double) * (size_t)k);
     if (b > OR_TAU * (amax + bmax)) {
       beta[j + 1] = b;

@@ -35,7 +35,7 @@ pl_status hip_fail(hipError_t e, const char *what) {
               hipGetErrorString(e));
 }
 
-constexpr int kMaxK = 64;
+constexpr int kMaxK = 128;
 
 int group_for(int k) {
   int g = 8;
@@ -44,10 +44,10 @@ int group_for(int k) {
 }
 
 // tuned LDS row padding per lane-group size (index log2(group) - 3)
-int g_tuned_pad[4] = {4, 4, 4, 4};
+int g_tuned_pad[5] = {4, 4, 4, 4, 4};
 
 int tuned_pad(int group) {
-  const int i = group == 8 ? 0 : group == 16 ? 1 : group == 32 ? 2 : 3;
+  const int i = group == 8 ? 0 : group == 16 ? 1 : group == 32 ? 2 : group == 64 ? 3 : 4;
   return g_tuned_pad[i];
 }
 
@@ -133,7 +133,8 @@ pl_status run_solve(const float *G, int64_t n, int64_t ld, const int32_t *idx,
 
 pl_status pick_form(int form, int k, int &group, int &pad) {
   if (form == PL_FORM_PLAIN) {
-    group = 64;
+    // one 64-lane wavefront per subset; a 128-lane workgroup beyond k = 64
+    group = k <= 64 ? 64 : 128;
     pad = 4;
   } else if (form == PL_FORM_TUNED) {
     group = group_for(k);

@@ -17,8 +17,9 @@
 //  (b) k Lanczos steps (ParaLanczos.cu:1285-1374) in FP32: matvec from the
 //      VGPR row; alpha = v^T G v by a compensated (TwoProduct/TwoSum) group
 //      reduction; three-term update; one full classical Gram-Schmidt pass
-//      against all previous vectors; beta = ||w|| compensated; deterministic
-//      restart on breakdown. The all-to-all broadcasts of v, w and the
+//      against all previous vectors, repeated when it removed more than half
+//      of w's norm (Daniel, Gragg, Kaufman, Stewart, Math. Comp. 1976);
+//      beta = ||w|| compensated; deterministic restart on breakdown. The all-to-all broadcasts of v, w and the
 //      Gram-Schmidt coefficients use DPP row broadcasts fused into v_fmac for
 //      groups of 16/32/64 lanes (v_permlane16/32_swap bring the other rows'
 //      values into each row), and LDS for groups of 8.
@@ -207,9 +208,9 @@ __device__ __forceinline__ void comp_step(float &p, float &e, float po, float eo
 }
 
 template <int GROUP>
-__device__ __forceinline__ float g_dot2(float x, float y) {
-  float p = x * y;
-  float e = __builtin_fmaf(x, y, -p);
+__device__ __forceinline__ void g_dot2_pe(float x, float y, float &p, float &e) {
+  p = x * y;
+  e = __builtin_fmaf(x, y, -p);
   if constexpr (GROUP < 16) {
 #pragma unroll
     for (int m = 1; m < GROUP; m <<= 1) comp_step(p, e, __shfl_xor(p, m, GROUP), __shfl_xor(e, m, GROUP));
@@ -235,6 +236,12 @@ __device__ __forceinline__ float g_dot2(float x, float y) {
       comp_step(p, e, pb, eb);
     }
   }
+}
+
+template <int GROUP>
+__device__ __forceinline__ float g_dot2(float x, float y) {
+  float p, e;
+  g_dot2_pe<GROUP>(x, y, p, e);
   return p + e;
 }
 
@@ -330,9 +337,15 @@ __device__ uint32_t lanczos_lds(const float (&g)[GROUP], int k, int gl, float *R
     my_a = mine ? a : my_a;
     amax = fmaxf(amax, fabsf(a));
     w = __builtin_fmaf(-a, vi, __builtin_fmaf(-bj, vp, w));
+    const float pre = g_sum<GROUP>(w * w);
     w = cgs_pass_lds<GROUP>(w, vrow, vcol, Lw, Lh, gl);
     if (j == k - 1) break;
-    const float b = sqrtf(g_dot2<GROUP>(w, w));
+    float b2 = g_dot2<GROUP>(w, w);
+    if (b2 < 0.5f * pre) {  // DGKS: w was mostly in span(V); orthogonalise again
+      w = cgs_pass_lds<GROUP>(w, vrow, vcol, Lw, Lh, gl);
+      b2 = g_dot2<GROUP>(w, w);
+    }
+    const float b = sqrtf(b2);
     vp = vi;
     float scale;
     if (b > TAU * (amax + bmax)) {
@@ -424,10 +437,16 @@ __device__ uint32_t lanczos_dpp(const float (&g)[GROUP], int k, int gl, float *R
     my_a = (gl == j) ? a : my_a;
     amax = fmaxf(amax, fabsf(a));
     w = __builtin_fmaf(-a, vi, __builtin_fmaf(-bj, vp, w));
+    const float pre = g_sum_dpp<GROUP>(w * w);
     wave_sync();
     w = cgs_pass_dpp<GROUP, STRIDE>(w, V, gl);
     if (j == k - 1) break;
-    const float b = sqrtf(g_dot2<GROUP>(w, w));
+    float b2 = g_dot2<GROUP>(w, w);
+    if (b2 < 0.5f * pre) {  // DGKS: w was mostly in span(V); orthogonalise again
+      w = cgs_pass_dpp<GROUP, STRIDE>(w, V, gl);
+      b2 = g_dot2<GROUP>(w, w);
+    }
+    const float b = sqrtf(b2);
     vp = vi;
     float scale;
     if (b > TAU * (amax + bmax)) {
@@ -600,6 +619,279 @@ void solve_kernel(SolveArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wide subsets (64 < k <= 128): one 128-lane workgroup (two wavefronts) per
+// subset. Lane i owns row i of G_S in VGPRs; the padded LDS tile that receives
+// the gather becomes the Lanczos basis V (V[i*S + l] = entry i of v_l), read
+// by rows (b128) and by columns (conflict-free b32); the broadcasts of v, w
+// and the Gram-Schmidt coefficients and the cross-wave halves of every
+// reduction go through LDS with workgroup barriers.
+// ---------------------------------------------------------------------------
+constexpr int WIDE = 128;
+
+struct WideShared {
+  float lv[WIDE], lw[WIDE], lh[WIDE];
+  float red[8];
+  double redd[8];
+  int ired[4];
+};
+
+// compensated dot over the workgroup: per-wave Dot2, then the two wave
+// partials combined in wave order (every lane gets identical bits)
+__device__ __forceinline__ float wide_dot2(float x, float y, WideShared &sh) {
+  float p, e;
+  g_dot2_pe<64>(x, y, p, e);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    sh.red[2 * w] = p;
+    sh.red[2 * w + 1] = e;
+  }
+  __syncthreads();
+  float p0 = sh.red[0], e0 = sh.red[1];
+  comp_step(p0, e0, sh.red[2], sh.red[3]);
+  return p0 + e0;
+}
+
+__device__ __forceinline__ float wide_sum(float x, WideShared &sh) {
+  x = g_sum_dpp<64>(x);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh.red[4 + (threadIdx.x >> 6)] = x;
+  __syncthreads();
+  return sh.red[4] + sh.red[5];
+}
+
+__device__ __forceinline__ double wide_min(double x, WideShared &sh) {
+  x = g_min<64>(x);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh.redd[threadIdx.x >> 6] = x;
+  __syncthreads();
+  return fmin(sh.redd[0], sh.redd[1]);
+}
+
+__device__ __forceinline__ double wide_max(double x, WideShared &sh) {
+  x = g_max<64>(x);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh.redd[2 + (threadIdx.x >> 6)] = x;
+  __syncthreads();
+  return fmax(sh.redd[2], sh.redd[3]);
+}
+
+// sum_c x[c] * L[c] over a 16-byte aligned LDS vector read by broadcast
+// (two accumulators split by index parity)
+template <int N>
+__device__ __forceinline__ float lds_dot(const float *L, const float (&x)[N]) {
+  float acc0 = 0.f, acc1 = 0.f;
+#pragma unroll
+  for (int q = 0; q < N / 4; ++q) {
+    const float4 t = reinterpret_cast<const float4 *>(L)[q];
+    acc0 = __builtin_fmaf(x[4 * q + 0], t.x, acc0);
+    acc1 = __builtin_fmaf(x[4 * q + 1], t.y, acc1);
+    acc0 = __builtin_fmaf(x[4 * q + 2], t.z, acc0);
+    acc1 = __builtin_fmaf(x[4 * q + 3], t.w, acc1);
+  }
+  return acc0 + acc1;
+}
+
+// Gram-Schmidt pass against the LDS basis: lane l forms h_l = v_l . w from
+// column l of V, lane i subtracts sum_l V[i][l] h_l from row i
+template <int S>
+__device__ __forceinline__ float wide_cgs(float w, const float *V, WideShared &sh) {
+  const int gl = threadIdx.x;
+  sh.lw[gl] = w;
+  __syncthreads();
+  float acc0 = 0.f, acc1 = 0.f;
+#pragma unroll 8
+  for (int q = 0; q < WIDE / 4; ++q) {
+    const float4 t = reinterpret_cast<const float4 *>(sh.lw)[q];
+    acc0 = __builtin_fmaf(V[(4 * q + 0) * S + gl], t.x, acc0);
+    acc1 = __builtin_fmaf(V[(4 * q + 1) * S + gl], t.y, acc1);
+    acc0 = __builtin_fmaf(V[(4 * q + 2) * S + gl], t.z, acc0);
+    acc1 = __builtin_fmaf(V[(4 * q + 3) * S + gl], t.w, acc1);
+  }
+  sh.lh[gl] = acc0 + acc1;
+  __syncthreads();
+  acc0 = 0.f;
+  acc1 = 0.f;
+#pragma unroll 8
+  for (int q = 0; q < WIDE / 4; ++q) {
+    const float4 t = reinterpret_cast<const float4 *>(sh.lh)[q];
+    float4 r;
+    if constexpr (S % 4 == 0) {
+      r = reinterpret_cast<const float4 *>(V + gl * S)[q];
+    } else {
+      r = make_float4(V[gl * S + 4 * q], V[gl * S + 4 * q + 1], V[gl * S + 4 * q + 2],
+                      V[gl * S + 4 * q + 3]);
+    }
+    acc0 = __builtin_fmaf(r.x, t.x, acc0);
+    acc1 = __builtin_fmaf(r.y, t.y, acc1);
+    acc0 = __builtin_fmaf(r.z, t.z, acc0);
+    acc1 = __builtin_fmaf(r.w, t.w, acc1);
+  }
+  return w - (acc0 + acc1);
+}
+
+template <int PAD, int MODE>
+__global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
+  constexpr int S = WIDE + PAD;
+  __shared__ __attribute__((aligned(16))) float V[WIDE * S];
+  __shared__ __attribute__((aligned(16))) WideShared sh;
+  const int gl = threadIdx.x;
+  const int64_t slot = blockIdx.x;
+  if (slot >= a.nslots) return;  // uniform over the workgroup
+  const int64_t s = a.order ? (int64_t)a.order[slot] : slot;
+  int k;
+  int64_t off;
+  if (a.offsets) {
+    off = a.offsets[s];
+    k = (int)(a.offsets[s + 1] - off);
+  } else {
+    k = a.kuni;
+    off = s * (int64_t)k;
+  }
+  uint32_t st = (k < 1 || k > WIDE) ? ST_BADK : 0u;
+  int my = 0;
+  if (!st && gl < k) {
+    my = a.idx[off + gl];
+    if (my < 0 || (int64_t)my >= a.n) st = ST_BADINDEX;
+  }
+  if (__syncthreads_or(st != 0)) st |= (k < 1 || k > WIDE) ? ST_BADK : ST_BADINDEX;
+  int *lidx = reinterpret_cast<int *>(sh.lv);
+  lidx[gl] = my;
+  __syncthreads();
+
+  // ---- (a) gather: row r per step, lanes = columns c >= r, 8 loads in flight ----
+  bool fin = true;
+  for (int r0 = 0; r0 < WIDE; r0 += 8) {
+    float x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int r = r0 + u;
+      x[u] = 0.f;
+      if (st == 0 && r < k && gl < k && gl >= r) {
+        const int ir = lidx[r];
+        const int64_t lo = min(ir, my), hi = max(ir, my);
+        x[u] = a.G[lo * a.ld + hi];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int r = r0 + u;
+      fin = fin && isfinite(x[u]);
+      if (gl >= r) {
+        V[r * S + gl] = x[u];
+        V[gl * S + r] = x[u];
+      }
+    }
+  }
+  if (__syncthreads_or(!fin)) st |= ST_NONFINITE;
+  float g[WIDE];
+#pragma unroll
+  for (int c = 0; c < WIDE; ++c) g[c] = V[gl * S + c];
+  __syncthreads();
+
+  if constexpr (MODE == MODE_DUMP) {
+    if (gl < k && st == 0) {
+      float *o = a.gs_out + s * (int64_t)k * k + (int64_t)gl * k;
+#pragma unroll
+      for (int c = 0; c < WIDE; ++c)
+        if (c < k) o[c] = g[c];
+    }
+    return;
+  } else if constexpr (MODE == MODE_CHECKSUM) {
+    float acc = 0.f;
+#pragma unroll
+    for (int c = 0; c < WIDE; ++c) acc += g[c];
+    acc = g_sum<64>(acc);
+    __syncthreads();
+    if ((gl & 63) == 0) sh.red[gl >> 6] = acc;
+    __syncthreads();
+    if (gl == 0) a.gs_out[s] = st ? NAN : sh.red[0] + sh.red[1];
+    return;
+  } else {
+    constexpr uint32_t FATAL = ST_BADK | ST_BADINDEX | ST_NONFINITE;
+    if (!(st & FATAL)) {
+      // ---- (b) Lanczos recurrence, ParaLanczos.cu:1285-1374 with the build's
+      // substitutions (ones/sqrt(k) start, full reorthogonalisation, restart) ----
+      float vi = (gl < k) ? (1.0f / sqrtf((float)k)) : 0.f;
+#pragma unroll
+      for (int c = 0; c < WIDE; ++c) V[gl * S + c] = (c == 0) ? vi : 0.f;
+      float vp = 0.f, bj = 0.f, amax = 0.f, bmax = 0.f, my_a = 0.f, my_b = 0.f;
+      for (int j = 0; j < k; ++j) {
+        sh.lv[gl] = vi;
+        __syncthreads();
+        float w = lds_dot<WIDE>(sh.lv, g);
+        const float al = wide_dot2(vi, w, sh);
+        my_a = (gl == j) ? al : my_a;
+        amax = fmaxf(amax, fabsf(al));
+        w = __builtin_fmaf(-al, vi, __builtin_fmaf(-bj, vp, w));
+        const float pre = wide_sum(w * w, sh);
+        w = wide_cgs<S>(w, V, sh);
+        if (j == k - 1) break;
+        float b2 = wide_dot2(w, w, sh);
+        if (b2 < 0.5f * pre) {  // DGKS: w was mostly in span(V); orthogonalise again
+          w = wide_cgs<S>(w, V, sh);
+          b2 = wide_dot2(w, w, sh);
+        }
+        const float b = sqrtf(b2);
+        vp = vi;
+        float scale;
+        if (b > TAU * (amax + bmax)) {
+          bj = b;
+          bmax = fmaxf(bmax, b);
+          scale = 1.0f / b;
+        } else {
+          // restart: e_i for the first i of largest residual 1 - ||V[i][:]||^2,
+          // orthogonalised twice against the basis; beta_{j+1} = 0
+          st |= ST_BREAKDOWN;
+          bj = 0.f;
+          float res = 1.f;
+          for (int l = 0; l < WIDE; ++l) res = __builtin_fmaf(-V[gl * S + l], V[gl * S + l], res);
+          const int wb = restart_lane<64>(res, gl, k);
+          const float wv = __shfl(gl < k ? res : -1.f, wb & 63, 64);
+          __syncthreads();
+          if ((gl & 63) == 0) {
+            sh.red[gl >> 6] = wv;
+            sh.ired[gl >> 6] = wb;
+          }
+          __syncthreads();
+          const int bi = (sh.red[1] > sh.red[0]) ? sh.ired[1] : sh.ired[0];
+          w = (gl == bi) ? 1.f : 0.f;
+          w = wide_cgs<S>(w, V, sh);
+          w = wide_cgs<S>(w, V, sh);
+          scale = 1.0f / sqrtf(wide_dot2(w, w, sh));
+        }
+        vi = w * scale;
+        V[gl * S + j + 1] = vi;
+        my_b = (gl == j + 1) ? bj : my_b;
+      }
+      const bool finite = (gl >= k) || (isfinite(my_a) && isfinite(my_b));
+      if (__syncthreads_or(!finite)) st |= ST_NONFINITE;
+      if (!(st & FATAL)) {
+        // T and its bracket data to the workspace
+        float2 *tri = a.tri + s * (int64_t)a.tstride;
+        sh.lw[gl] = my_b;
+        __syncthreads();
+        const double al = my_a, b = my_b, bn = (gl + 1 < WIDE) ? (double)sh.lw[gl + 1] : 0.0;
+        double glo = INFINITY, ghi = -INFINITY, amin = INFINITY, amx = -INFINITY;
+        if (gl < k) {
+          tri[gl] = make_float2(my_a, gl > 0 ? my_b : 0.f);
+          const double r = (gl > 0 ? fabs(b) : 0.0) + (gl + 1 < k ? fabs(bn) : 0.0);
+          glo = al - r;
+          ghi = al + r;
+          amin = amx = al;
+        }
+        glo = wide_min(glo, sh);
+        ghi = wide_max(ghi, sh);
+        amin = wide_min(amin, sh);
+        amx = wide_max(amx, sh);
+        if (gl == 0) a.bounds[s] = make_double4(glo, ghi, amin, amx);
+      }
+    }
+    if (gl == 0) a.status[s] = st;
+  }
+}
+
 using SolveFn = void (*)(SolveArgs);
 
 template <int GROUP, int PAD>
@@ -609,6 +901,15 @@ struct Variant {
 };
 
 static SolveFn pick(int group, int pad, int mode) {
+  if (group == WIDE) {
+    static constexpr SolveFn w1[3] = {solve_wide_kernel<1, 0>, solve_wide_kernel<1, 1>,
+                                      solve_wide_kernel<1, 2>};
+    static constexpr SolveFn w4[3] = {solve_wide_kernel<4, 0>, solve_wide_kernel<4, 1>,
+                                      solve_wide_kernel<4, 2>};
+    if (pad == 1) return w1[mode];
+    if (pad == 4) return w4[mode];
+    return nullptr;
+  }
 #define PL_CASE(G, P) \
   if (group == G && pad == P) return Variant<G, P>::fns[mode];
   PL_CASE(8, 0) PL_CASE(8, 1) PL_CASE(8, 4)
@@ -626,10 +927,11 @@ hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStr
   SolveFn fn = pick(group, pad, mode);
   if (!fn) return hipErrorInvalidValue;
   if (a.nslots <= 0) return hipSuccess;
-  const int spb = BLOCK / group;
+  const int threads = group == WIDE ? WIDE : BLOCK;
+  const int spb = group == WIDE ? 1 : BLOCK / group;
   const int64_t blocks = (a.nslots + spb - 1) / spb;
   if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(BLOCK), 0, s, a);
+  hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(threads), 0, s, a);
   return hipGetLastError();
 }
 
@@ -853,6 +1155,11 @@ hipError_t launch_bisect(int mode, const BisectArgs &a, hipStream_t s) {
   if (mode == 0) {
     const int64_t blocks = (a.nsub + BIS_BLOCK - 1) / BIS_BLOCK;
     const size_t lds = (size_t)BIS_BLOCK * (a.tstride + 1) * sizeof(float2);
+    if (lds > 65536) {
+      hipError_t e = hipFuncSetAttribute((const void *)bisect_extreme_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(bisect_extreme_kernel, dim3((unsigned)blocks), dim3(BIS_BLOCK), lds, s, a);
   } else {
     const int64_t threads = a.nsub * a.kuni;
@@ -860,6 +1167,11 @@ hipError_t launch_bisect(int mode, const BisectArgs &a, hipStream_t s) {
     // a block's 64 targets span at most 64/k + 2 subsets
     const int rows = BIS_BLOCK / a.kuni + 2;
     const size_t lds = (size_t)rows * (a.tstride + 1) * sizeof(float2);
+    if (lds > 65536) {
+      hipError_t e = hipFuncSetAttribute((const void *)bisect_all_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(bisect_all_kernel, dim3((unsigned)blocks), dim3(BIS_BLOCK), lds, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
