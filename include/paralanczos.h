@@ -45,8 +45,14 @@ typedef enum {
 #define PL_SUB_BADK 16u     /* subset size outside the supported range    */
 
 /* solve forms */
-#define PL_FORM_TUNED 0 /* lane group = smallest of 8/16/32/64 >= k, tuned LDS pad */
-#define PL_FORM_PLAIN 1 /* one 64-lane wavefront per subset, one launch per k      */
+#define PL_FORM_TUNED 0 /* lane group and LDS pad from the tuning table; FP64
+                           Gram-Schmidt coefficients from k = 128 on            */
+#define PL_FORM_PLAIN 1 /* one 64-lane wavefront per subset (128 / 512 lanes
+                           beyond k = 64 / 128), FP32 throughout, one launch per k */
+/* precision flags, OR-ed into the form: FP32 or FP64 Gram-Schmidt
+   coefficients at every k > 64 (subsets of k <= 64 always use FP32) */
+#define PL_FORM_REORTH_F32 0x100
+#define PL_FORM_REORTH_F64 0x200
 
 const char *pl_version(void);
 const char *pl_status_string(pl_status st);
@@ -66,7 +72,11 @@ int pl_max_k(void);
  * triangle G[min(i,j)][max(i,j)] is read).
  *   offsets == NULL: S_s = idx[s*k .. s*k+k-1] (all subsets of size k);
  *   offsets != NULL: S_s = idx[offsets[s] .. offsets[s+1]-1] (int64 offsets,
- *                    device memory) and k is the largest subset size.
+ *                    device memory) and k is the largest subset size; the
+ *                    tuned form bins the subsets by size on the device and
+ *                    drains each bin with one persistent launch, the plain
+ *                    form makes one launch per distinct size (and waits for
+ *                    the bin counts on the host).
  * Outputs: lmin, lmax, delta (FP32) and status (PL_SUB_* bits) per subset.
  * Subsets with status BADINDEX/BADK/NONFINITE get NaN eigenvalues.
  */
@@ -93,6 +103,42 @@ pl_status pl_solve_variant(const float *G, int64_t n, int64_t ld, const int32_t 
                            const int64_t *offsets, int k, int64_t nsub, float *lmin,
                            float *lmax, float *delta, uint32_t *status, int lanes, int pad,
                            void *stream);
+
+/*
+ * Plans (no reference counterpart: the reference re-uploads G and relaunches
+ * per call, ParaLanczos.cu:1837-1858). A plan binds one solve to fixed device
+ * buffers and owns its workspace (allocated by pl_plan_create, nothing is
+ * allocated afterwards). The first pl_plan_execute of a plan whose work needs
+ * no host synchronisation (every form except PL_FORM_PLAIN with offsets)
+ * records the launch sequence into a HIP graph; later executions replay it.
+ * The contents of idx, offsets and G may change between executions; subset
+ * sizes are re-binned on the device each time. Executions of one plan must
+ * not overlap; pl_plan_destroy waits for the device.
+ */
+typedef struct pl_plan pl_plan;
+pl_status pl_plan_create(pl_plan **plan, const float *G, int64_t n, int64_t ld,
+                         const int32_t *idx, const int64_t *offsets, int k, int64_t nsub,
+                         float *lmin, float *lmax, float *delta, uint32_t *status, int form);
+pl_status pl_plan_execute(pl_plan *plan, void *stream);
+int pl_plan_is_graph(const pl_plan *plan); /* 1 once the plan replays a graph */
+pl_status pl_plan_destroy(pl_plan *plan);
+
+/*
+ * Tuning table of PL_FORM_TUNED: the variant (lanes per subset, LDS row
+ * padding) used for each k; mixed batches are binned by runs of equal
+ * variants. pl_autotune times every variant with lanes >= k (lane groups of
+ * 8..128, which give bit-identical results, and the 512-lane kernel beyond
+ * k = 128) and pad 0/1/4 on
+ * `sample` seeded subsets of each size in klist (nk sizes, each <= n), keeps
+ * the fastest, and gives every unlisted size the winner of the next listed
+ * size above it. best_ms (nk entries, may be NULL) receives the winning
+ * times. Synchronous; the table is shared by all threads of the process.
+ */
+pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klist, int nk,
+                      int64_t sample, uint64_t seed, double *best_ms, void *stream);
+pl_status pl_tuning_get(int k, int *lanes, int *pad);
+pl_status pl_tuning_set(int k, int lanes, int pad);
+pl_status pl_tuning_reset(void);
 
 /* all k eigenvalues ascending eig[s*k + j] (FP64), det[s] = prod_j eig, and
    err[s] = per-subset status, for nsub subsets of size k (uniform) */

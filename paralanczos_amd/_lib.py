@@ -27,6 +27,14 @@ SIGNATURES = {
     "pl_solve_ws": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, P, ctypes.c_size_t, P]),
     "pl_solve_variant": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, INT, P]),
     "pl_solve_alleig": (ST, [P, I64, I64, P, INT, I64, P, P, P, P]),
+    "pl_plan_create": (ST, [P, P, I64, I64, P, P, INT, I64, P, P, P, P, INT]),
+    "pl_plan_execute": (ST, [P, P]),
+    "pl_plan_is_graph": (INT, [P]),
+    "pl_plan_destroy": (ST, [P]),
+    "pl_autotune": (ST, [P, I64, I64, P, INT, I64, U64, P, P]),
+    "pl_tuning_get": (ST, [INT, P, P]),
+    "pl_tuning_set": (ST, [INT, INT, INT]),
+    "pl_tuning_reset": (ST, []),
     "pl_gather": (ST, [P, I64, I64, P, INT, I64, P, INT, INT, INT, P]),
     "pl_gen_subsets": (ST, [U64, I64, I64, INT, INT, P, P]),
     "pl_gen_subsets_csr": (ST, [U64, I64, I64, P, INT, P, P]),

@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -35,7 +36,7 @@ pl_status hip_fail(hipError_t e, const char *what) {
               hipGetErrorString(e));
 }
 
-constexpr int kMaxK = 128;
+constexpr int kMaxK = pl::KMAX_ALL;
 
 int group_for(int k) {
   int g = 8;
@@ -43,12 +44,95 @@ int group_for(int k) {
   return g;
 }
 
-// tuned LDS row padding per lane-group size (index log2(group) - 3)
-int g_tuned_pad[5] = {4, 4, 4, 4, 4};
+// ---------------------------------------------------------------------------
+// Tuning table: the solve variant (lanes per subset, LDS row padding) used for
+// each k by the tuned form. Defaults: the smallest lane group >= k, pad 4;
+// pl_autotune overwrites entries with measured winners.
+// ---------------------------------------------------------------------------
+struct Variant {
+  int lanes, pad;
+  bool f64 = false;  // FP64 Gram-Schmidt coefficients
+};
+std::mutex g_tune_mu;
+Variant g_tune[pl::KMAX_ALL + 1];
+bool g_tune_ready = false;
 
-int tuned_pad(int group) {
-  const int i = group == 8 ? 0 : group == 16 ? 1 : group == 32 ? 2 : group == 64 ? 3 : 4;
-  return g_tuned_pad[i];
+Variant default_variant(int k) { return Variant{group_for(k), 4, false}; }
+
+constexpr int kFormMask = 0xff;
+constexpr int kPrecMask = PL_FORM_REORTH_F32 | PL_FORM_REORTH_F64;
+
+// FP64 Gram-Schmidt coefficients: forced by a flag, otherwise the tuned form
+// uses them from k = 128 on (lane groups of 128 and 256 carry them)
+bool use_f64(int form, int k, int lanes) {
+  if (lanes < 128) return false;
+  if (form & PL_FORM_REORTH_F64) return true;
+  if (form & PL_FORM_REORTH_F32) return false;
+  return (form & kFormMask) == PL_FORM_TUNED && k >= 128;
+}
+
+void tune_init_locked() {
+  if (g_tune_ready) return;
+  for (int k = 1; k <= pl::KMAX_ALL; ++k) g_tune[k] = default_variant(k);
+  g_tune_ready = true;
+}
+
+Variant tuned_variant(int k) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  tune_init_locked();
+  return g_tune[k];
+}
+
+Variant plain_variant(int k) {
+  // one 64-lane wavefront per subset; a 128-lane workgroup beyond k = 64 and
+  // a 512-lane one beyond k = 128
+  return Variant{k <= 64 ? 64 : k <= 128 ? 128 : 256, 4, false};
+}
+
+// Bins of a mixed-size batch. Tuned: maximal runs of consecutive k sharing a
+// variant (the bin edges follow the tuning table). Plain: one bin per exact k.
+// Bin 0 also takes every size outside [1, kmax]; its launch flags them BADK.
+struct BinPlan {
+  pl::BinLut lut;
+  int nbins = 0;
+  Variant var[pl::MAXBIN];
+  int kbin[pl::MAXBIN];  // plain: the k of the bin (0 for the invalid sizes)
+};
+
+Variant job_variant(int form, int k) {
+  Variant v = (form & kFormMask) == PL_FORM_PLAIN ? plain_variant(k) : tuned_variant(k);
+  v.f64 = use_f64(form, k, v.lanes);
+  return v;
+}
+
+void make_bins(int form, int kmax, BinPlan &bp) {
+  bp.nbins = 0;
+  if ((form & kFormMask) == PL_FORM_PLAIN) {
+    bp.var[0] = job_variant(form, 1);
+    bp.kbin[0] = 0;
+    for (int k = 1; k <= kmax; ++k) {
+      bp.var[k] = job_variant(form, k);
+      bp.kbin[k] = k;
+    }
+    bp.nbins = kmax + 1;
+    for (int i = 0; i < pl::KLUT; ++i) bp.lut.bin[i] = (i >= 1 && i <= kmax) ? (uint16_t)i : 0;
+    return;
+  }
+  int prev_l = -1, prev_p = -1, prev_f = -1;
+  uint16_t cur = 0;
+  for (int i = 0; i < pl::KLUT; ++i) bp.lut.bin[i] = 0;
+  for (int k = 1; k <= kmax; ++k) {
+    const Variant v = job_variant(form, k);
+    if (v.lanes != prev_l || v.pad != prev_p || (int)v.f64 != prev_f) {
+      cur = (uint16_t)bp.nbins++;
+      bp.var[cur] = v;
+      bp.kbin[cur] = k;
+      prev_l = v.lanes;
+      prev_p = v.pad;
+      prev_f = v.f64;
+    }
+    bp.lut.bin[k] = cur;
+  }
 }
 
 pl_status check_solve_args(const float *G, int64_t n, int64_t ld, const int32_t *idx, int k,
@@ -56,7 +140,8 @@ pl_status check_solve_args(const float *G, int64_t n, int64_t ld, const int32_t 
   if (!G || !idx) return fail(PL_ERR_ARG, "G and idx must be non-null");
   if (n < 1 || n > 0x7fffffffLL) return fail(PL_ERR_ARG, "n = %lld out of range", (long long)n);
   if (ld < n) return fail(PL_ERR_ARG, "ld = %lld < n = %lld", (long long)ld, (long long)n);
-  if (nsub < 0) return fail(PL_ERR_ARG, "nsub = %lld < 0", (long long)nsub);
+  if (nsub < 0 || nsub > 0x7fffffffLL)
+    return fail(PL_ERR_ARG, "nsub = %lld outside [0, 2^31)", (long long)nsub);
   if (k < 1) return fail(PL_ERR_ARG, "k = %d < 1", k);
   if (k > kMaxK) return fail(PL_ERR_UNSUPPORTED, "k = %d > %d is not supported", k, kMaxK);
   return PL_OK;
@@ -85,64 +170,145 @@ struct Workspace {
   }
 };
 
-pl_status run_solve(const float *G, int64_t n, int64_t ld, const int32_t *idx,
-                    const int64_t *offsets, int k, int64_t nsub, float *lmin, float *lmax,
-                    float *delta, uint32_t *status, int group, int pad, void *ws,
-                    size_t ws_bytes, void *stream) {
-  if (!lmin || !lmax || !delta || !status)
-    return fail(PL_ERR_ARG, "output pointers must be non-null");
-  if (group < k) return fail(PL_ERR_ARG, "lanes per subset %d < k = %d", group, k);
-  if (!pl::solve_variant_exists(group, pad))
-    return fail(PL_ERR_ARG, "no variant with %d lanes and pad %d", group, pad);
-  if (nsub == 0) return PL_OK;
-  hipStream_t s = (hipStream_t)stream;
-  Workspace w;
-  pl_status st = w.acquire(ws, ws_bytes, pl::workspace_bytes(nsub, k), s);
-  if (st != PL_OK) return st;
-  float2 *tri = (float2 *)w.p;
-  double4 *bounds = (double4 *)((char *)w.p + pl::tridiag_bytes(nsub, k));
+// everything one solve call needs, bound to device buffers
+struct SolveJob {
+  const float *G;
+  int64_t n, ld;
+  const int32_t *idx;
+  const int64_t *offsets;
+  int k;  // uniform size, or the largest size of a mixed batch
+  int64_t nsub;
+  float *lmin, *lmax, *delta;
+  uint32_t *status;
+  int form;           // PL_FORM_TUNED / PL_FORM_PLAIN (| precision flags), or -1 for an
+                      // explicit variant
+  Variant explicit_v;  // used when form == -1
+};
+
+pl::SolveArgs solve_args(const SolveJob &j, const pl::WorkspaceParts &w) {
   pl::SolveArgs a{};
-  a.G = G;
-  a.n = n;
-  a.ld = ld;
-  a.idx = idx;
-  a.offsets = offsets;
-  a.kuni = offsets ? 0 : k;
-  a.nslots = nsub;
-  a.status = status;
-  a.tri = tri;
-  a.tstride = k;
-  a.bounds = bounds;
-  hipError_t e = pl::launch_solve(group, pad, pl::MODE_TRIDIAG, a, s);
-  if (e != hipSuccess) return hip_fail(e, "solve launch");
+  a.G = j.G;
+  a.n = j.n;
+  a.ld = j.ld;
+  a.idx = j.idx;
+  a.offsets = j.offsets;
+  a.kuni = j.offsets ? 0 : j.k;
+  a.kcap = j.k;
+  a.nslots = j.nsub;
+  a.status = j.status;
+  a.tri = w.tri;
+  a.tstride = j.k;
+  a.bounds = w.bounds;
+  return a;
+}
+
+pl_status run_bisect(const SolveJob &j, const pl::WorkspaceParts &w, const int32_t *order,
+                     hipStream_t s) {
   pl::BisectArgs b{};
-  b.tri = tri;
-  b.tstride = k;
-  b.bounds = bounds;
-  b.offsets = offsets;
-  b.kuni = offsets ? 0 : k;
-  b.nsub = nsub;
-  b.status = status;
-  b.lmin = lmin;
-  b.lmax = lmax;
-  b.delta = delta;
-  e = pl::launch_bisect(0, b, s);
+  b.order = order;
+  b.tri = w.tri;
+  b.tstride = j.k;
+  b.bounds = w.bounds;
+  b.offsets = j.offsets;
+  b.kuni = j.offsets ? 0 : j.k;
+  b.nsub = j.nsub;
+  b.status = j.status;
+  b.lmin = j.lmin;
+  b.lmax = j.lmax;
+  b.delta = j.delta;
+  hipError_t e = pl::launch_bisect(0, b, s);
   if (e != hipSuccess) return hip_fail(e, "bisection launch");
   return PL_OK;
 }
 
-pl_status pick_form(int form, int k, int &group, int &pad) {
-  if (form == PL_FORM_PLAIN) {
-    // one 64-lane wavefront per subset; a 128-lane workgroup beyond k = 64
-    group = k <= 64 ? 64 : 128;
-    pad = 4;
-  } else if (form == PL_FORM_TUNED) {
-    group = group_for(k);
-    pad = tuned_pad(group);
+// Whether a job runs without host synchronisation (and so can be captured
+// into a graph): all but the plain form of a mixed-size batch.
+bool job_is_async(const SolveJob &j) {
+  return !(j.offsets && j.form != -1 && (j.form & kFormMask) == PL_FORM_PLAIN);
+}
+
+// The stream-ordered work of one solve on a workspace of
+// pl::workspace_bytes(nsub, k) bytes.
+//   uniform k: one launch of the form's variant, then the bisection;
+//   mixed, tuned: device-side binning by variant, one persistent launch per
+//     bin draining its run of the order array, bisection in bin order;
+//   mixed, plain: binning by exact k, bin counts to the host, one launch per
+//     k sized to its count, bisection in bin order.
+pl_status run_job(const SolveJob &j, void *ws, hipStream_t s) {
+  const pl::WorkspaceParts w = pl::split_workspace(ws, j.nsub, j.k);
+  pl::SolveArgs a = solve_args(j, w);
+  hipError_t e;
+  if (!j.offsets) {
+    const Variant v = j.form == -1 ? j.explicit_v : job_variant(j.form, j.k);
+    e = pl::launch_solve(v.lanes, v.pad, pl::MODE_TRIDIAG, a, s, 0, v.f64);
+    if (e != hipSuccess) return hip_fail(e, "solve launch");
+    return run_bisect(j, w, nullptr, s);
+  }
+  if (j.form == -1) {  // explicit variant over a mixed batch: one launch, no binning
+    e = pl::launch_solve(j.explicit_v.lanes, j.explicit_v.pad, pl::MODE_TRIDIAG, a, s, 0,
+                         j.explicit_v.f64);
+    if (e != hipSuccess) return hip_fail(e, "solve launch");
+    return run_bisect(j, w, nullptr, s);
+  }
+  BinPlan bp;
+  make_bins(j.form, j.k, bp);
+  e = pl::launch_binning(j.offsets, 0, j.nsub, bp.lut, bp.nbins, w.table, w.order, s);
+  if (e != hipSuccess) return hip_fail(e, "binning launch");
+  a.order = w.order;
+  if ((j.form & kFormMask) == PL_FORM_TUNED) {
+    a.bins = w.table;
+    for (int b = 0; b < bp.nbins; ++b) {
+      a.bin = b;
+      e = pl::launch_solve(bp.var[b].lanes, bp.var[b].pad, pl::MODE_TRIDIAG, a, s, j.nsub,
+                           bp.var[b].f64);
+      if (e != hipSuccess) return hip_fail(e, "persistent solve launch");
+    }
   } else {
-    return fail(PL_ERR_ARG, "unknown form %d", form);
+    std::vector<int32_t> tab(2 * pl::MAXBIN);
+    e = hipMemcpyAsync(tab.data(), w.table, tab.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                       s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "bin counts to host");
+    for (int b = 0; b < bp.nbins; ++b) {
+      const int cnt = tab[pl::BIN_COUNT + b];
+      if (cnt == 0) continue;
+      a.order = w.order + tab[pl::BIN_START + b];
+      a.nslots = cnt;
+      e = pl::launch_solve(bp.var[b].lanes, bp.var[b].pad, pl::MODE_TRIDIAG, a, s, 0,
+                           bp.var[b].f64);
+      if (e != hipSuccess) return hip_fail(e, "per-k solve launch");
+    }
+  }
+  return run_bisect(j, w, w.order, s);
+}
+
+pl_status check_job(const SolveJob &j) {
+  pl_status st = check_solve_args(j.G, j.n, j.ld, j.idx, j.k, j.nsub);
+  if (st != PL_OK) return st;
+  if (!j.lmin || !j.lmax || !j.delta || !j.status)
+    return fail(PL_ERR_ARG, "output pointers must be non-null");
+  if (j.form == -1) {
+    if (j.explicit_v.lanes < j.k)
+      return fail(PL_ERR_ARG, "lanes per subset %d < k = %d", j.explicit_v.lanes, j.k);
+    if (!pl::solve_variant_exists(j.explicit_v.lanes, j.explicit_v.pad))
+      return fail(PL_ERR_ARG, "no variant with %d lanes and pad %d", j.explicit_v.lanes,
+                  j.explicit_v.pad);
+  } else if (((j.form & kFormMask) != PL_FORM_TUNED && (j.form & kFormMask) != PL_FORM_PLAIN) ||
+             (j.form & ~(kFormMask | kPrecMask)) || (j.form & kPrecMask) == kPrecMask) {
+    return fail(PL_ERR_ARG, "unknown form %d", j.form);
   }
   return PL_OK;
+}
+
+pl_status solve_once(const SolveJob &j, void *ws, size_t ws_bytes, void *stream) {
+  pl_status st = check_job(j);
+  if (st != PL_OK) return st;
+  if (j.nsub == 0) return PL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Workspace w;
+  st = w.acquire(ws, ws_bytes, pl::workspace_bytes(j.nsub, j.k), s);
+  if (st != PL_OK) return st;
+  return run_job(j, w.p, s);
 }
 
 }  // namespace
@@ -188,23 +354,223 @@ pl_status pl_solve_ws(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                       const int64_t *offsets, int k, int64_t nsub, float *lmin, float *lmax,
                       float *delta, uint32_t *status, int form, void *workspace,
                       size_t workspace_bytes, void *stream) {
-  pl_status st = check_solve_args(G, n, ld, idx, k, nsub);
-  if (st != PL_OK) return st;
-  int group, pad;
-  st = pick_form(form, k, group, pad);
-  if (st != PL_OK) return st;
-  return run_solve(G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, group, pad,
-                   workspace, workspace_bytes, stream);
+  SolveJob j{G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, form, {0, 0, false}};
+  return solve_once(j, workspace, workspace_bytes, stream);
 }
 
 pl_status pl_solve_variant(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                            const int64_t *offsets, int k, int64_t nsub, float *lmin,
                            float *lmax, float *delta, uint32_t *status, int lanes, int pad,
                            void *stream) {
-  pl_status st = check_solve_args(G, n, ld, idx, k, nsub);
+  SolveJob j{G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, -1, {lanes, pad, false}};
+  return solve_once(j, nullptr, 0, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Plans: a solve bound to fixed device buffers, with its own workspace. The
+// first execution of an asynchronous plan records its launch sequence into a
+// HIP graph; later executions replay the graph (the bins are recomputed on the
+// device from the offsets' current contents each time).
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct pl_plan {
+  SolveJob job;
+  void *ws = nullptr;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::mutex mu;
+};
+
+extern "C" {
+
+pl_status pl_plan_create(pl_plan **plan, const float *G, int64_t n, int64_t ld,
+                         const int32_t *idx, const int64_t *offsets, int k, int64_t nsub,
+                         float *lmin, float *lmax, float *delta, uint32_t *status, int form) {
+  if (!plan) return fail(PL_ERR_ARG, "plan must be non-null");
+  *plan = nullptr;
+  SolveJob j{G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, form, {0, 0, false}};
+  pl_status st = check_job(j);
   if (st != PL_OK) return st;
-  return run_solve(G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, lanes, pad,
-                   nullptr, 0, stream);
+  pl_plan *p = new pl_plan;
+  p->job = j;
+  hipError_t e = hipMalloc(&p->ws, pl::workspace_bytes(nsub > 0 ? nsub : 1, k));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    if (p->ws) (void)hipFree(p->ws);
+    delete p;
+    return hip_fail(e, "plan allocation");
+  }
+  *plan = p;
+  return PL_OK;
+}
+
+pl_status pl_plan_execute(pl_plan *p, void *stream) {
+  if (!p) return fail(PL_ERR_ARG, "plan must be non-null");
+  std::lock_guard<std::mutex> lock(p->mu);
+  hipStream_t s = (hipStream_t)stream;
+  if (p->job.nsub == 0) return PL_OK;
+  if (!job_is_async(p->job)) return run_job(p->job, p->ws, s);
+  if (!p->exec) {
+    hipError_t e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    pl_status st = run_job(p->job, p->ws, p->cap);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(p->cap, &g);
+    if (st != PL_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      p->exec = nullptr;
+      return hip_fail(e, "hipGraphInstantiate");
+    }
+    p->graph = g;
+  }
+  hipError_t e = hipGraphLaunch(p->exec, s);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+  return PL_OK;
+}
+
+int pl_plan_is_graph(const pl_plan *p) { return p && p->exec ? 1 : 0; }
+
+pl_status pl_plan_destroy(pl_plan *p) {
+  if (!p) return PL_OK;
+  hipError_t e = hipSuccess;
+  if (p->cap) e = hipStreamSynchronize(p->cap);
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->cap) (void)hipStreamDestroy(p->cap);
+  // the plan's workspace may still be in use by a replay on the caller's
+  // stream: wait for the device before releasing it
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (p->ws) (void)hipFree(p->ws);
+  delete p;
+  if (e != hipSuccess) return hip_fail(e, "plan teardown");
+  return PL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tuning table access and the autotuner
+// ---------------------------------------------------------------------------
+pl_status pl_tuning_get(int k, int *lanes, int *pad) {
+  if (k < 1 || k > kMaxK || !lanes || !pad) return fail(PL_ERR_ARG, "bad k or null output");
+  const Variant v = tuned_variant(k);
+  *lanes = v.lanes;
+  *pad = v.pad;
+  return PL_OK;
+}
+
+pl_status pl_tuning_set(int k, int lanes, int pad) {
+  if (k < 1 || k > kMaxK) return fail(PL_ERR_ARG, "k = %d outside [1, %d]", k, kMaxK);
+  if (lanes < k || !pl::solve_variant_exists(lanes, pad))
+    return fail(PL_ERR_ARG, "no variant with %d lanes and pad %d for k = %d", lanes, pad, k);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  tune_init_locked();
+  g_tune[k] = Variant{lanes, pad, false};
+  return PL_OK;
+}
+
+pl_status pl_tuning_reset(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune_ready = false;
+  tune_init_locked();
+  return PL_OK;
+}
+
+pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klist, int nk,
+                      int64_t sample, uint64_t seed, double *best_ms, void *stream) {
+  if (!G || !klist || nk < 1) return fail(PL_ERR_ARG, "G and klist must be non-null, nk >= 1");
+  if (sample < 1 || sample > (1 << 24)) return fail(PL_ERR_ARG, "sample outside [1, 2^24]");
+  int kmax = 0;
+  for (int i = 0; i < nk; ++i) {
+    if (klist[i] < 1 || klist[i] > kMaxK || klist[i] > n)
+      return fail(PL_ERR_ARG, "klist[%d] = %d outside [1, min(n, %d)]", i, klist[i], kMaxK);
+    kmax = std::max(kmax, klist[i]);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int32_t *idx = nullptr;
+  float *out = nullptr;
+  void *ws = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const size_t wsb = pl::workspace_bytes(sample, kmax);
+  hipError_t e = hipMalloc(&idx, (size_t)sample * kmax * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc(&out, (size_t)sample * 4 * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&ws, wsb);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  pl_status st = e == hipSuccess ? PL_OK : hip_fail(e, "autotune allocation");
+  static const int kLanes[] = {8, 16, 32, 64, 128, 256};
+  static const int kPads[] = {0, 1, 4};
+  for (int i = 0; i < nk && st == PL_OK; ++i) {
+    const int k = klist[i];
+    e = pl::launch_gen_subsets(seed, 0, sample, nullptr, k, (int)n, idx, s);
+    if (e != hipSuccess) {
+      st = hip_fail(e, "autotune subsets");
+      break;
+    }
+    Variant best{0, 0};
+    float best_t = 0.f;
+    for (int lanes : kLanes) {
+      // lane groups up to 128 compute bit-identical results; the 512-lane
+      // kernel (its own reduction order) is the one choice beyond k = 128
+      if (lanes < k || (lanes == 256 && k <= 128)) continue;
+      for (int pad : kPads) {
+        if (!pl::solve_variant_exists(lanes, pad)) continue;
+        SolveJob j{G,   n, ld, idx, nullptr, k, sample, out, out + sample, out + 2 * sample,
+                   (uint32_t *)(out + 3 * sample), -1,
+                   {lanes, pad, use_f64(PL_FORM_TUNED, k, lanes)}};
+        float t = 0.f;
+        for (int rep = 0; rep < 3 && st == PL_OK; ++rep) {  // rep 0 warms up
+          if ((e = hipEventRecord(e0, s)) != hipSuccess) st = hip_fail(e, "event");
+          if (st == PL_OK) st = run_job(j, ws, s);
+          if (st == PL_OK && (e = hipEventRecord(e1, s)) != hipSuccess) st = hip_fail(e, "event");
+          if (st == PL_OK && (e = hipEventSynchronize(e1)) != hipSuccess)
+            st = hip_fail(e, "autotune sync");
+          float ms = 0.f;
+          if (st == PL_OK && (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess)
+            st = hip_fail(e, "event time");
+          if (rep == 1 || (rep > 1 && ms < t)) t = ms;
+        }
+        if (st != PL_OK) break;
+        if (best.lanes == 0 || t < best_t) {
+          best = Variant{lanes, pad, false};
+          best_t = t;
+        }
+      }
+      if (st != PL_OK) break;
+    }
+    if (st != PL_OK) break;
+    {
+      std::lock_guard<std::mutex> lock(g_tune_mu);
+      tune_init_locked();
+      g_tune[k] = best;
+    }
+    if (best_ms) best_ms[i] = best_t;
+  }
+  if (st == PL_OK) {
+    // bin edges: a size between two measured ones takes the winner of the
+    // next measured size above it (whose lane count covers it)
+    std::vector<int> ks(klist, klist + nk);
+    std::sort(ks.begin(), ks.end());
+    ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    size_t q = 0;
+    for (int k = 1; k <= ks.back(); ++k) {
+      while (ks[q] < k) ++q;
+      g_tune[k] = g_tune[ks[q]];
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (ws) (void)hipFree(ws);
+  if (out) (void)hipFree(out);
+  if (idx) (void)hipFree(idx);
+  return st;
 }
 
 pl_status pl_solve_alleig(const float *G, int64_t n, int64_t ld, const int32_t *idx, int k,
@@ -217,21 +583,23 @@ pl_status pl_solve_alleig(const float *G, int64_t n, int64_t ld, const int32_t *
   Workspace w;
   st = w.acquire(nullptr, 0, pl::workspace_bytes(nsub, k), s);
   if (st != PL_OK) return st;
-  float2 *tri = (float2 *)w.p;
-  double4 *bounds = (double4 *)((char *)w.p + pl::tridiag_bytes(nsub, k));
-  const int group = group_for(k);
+  const pl::WorkspaceParts wp = pl::split_workspace(w.p, nsub, k);
+  float2 *tri = wp.tri;
+  double4 *bounds = wp.bounds;
+  const Variant v = job_variant(PL_FORM_TUNED, k);
   pl::SolveArgs a{};
   a.G = G;
   a.n = n;
   a.ld = ld;
   a.idx = idx;
   a.kuni = k;
+  a.kcap = k;
   a.nslots = nsub;
   a.status = (uint32_t *)err;
   a.tri = tri;
   a.tstride = k;
   a.bounds = bounds;
-  hipError_t e = pl::launch_solve(group, tuned_pad(group), pl::MODE_TRIDIAG, a, s);
+  hipError_t e = pl::launch_solve(v.lanes, v.pad, pl::MODE_TRIDIAG, a, s, 0, v.f64);
   if (e != hipSuccess) return hip_fail(e, "all-eigenvalue solve launch");
   pl::BisectArgs b{};
   b.tri = tri;
@@ -262,6 +630,7 @@ pl_status pl_gather(const float *G, int64_t n, int64_t ld, const int32_t *idx, i
   a.ld = ld;
   a.idx = idx;
   a.kuni = k;
+  a.kcap = k;
   a.nslots = nsub;
   a.gs_out = out;
   hipError_t e = pl::launch_solve(lanes, pad, mode == 0 ? pl::MODE_DUMP : pl::MODE_CHECKSUM, a,

@@ -13,14 +13,29 @@ enum Mode : int {
   MODE_CHECKSUM = 2,  // stage (a) only: sum of the G_S entries (FP32)
 };
 
+// Subset binning. Subsets are grouped by size k into bins (one bin per solve
+// variant, or one per exact k); the bin table in device memory holds, per bin,
+// its count, the start of its run in the order array and a fill cursor.
+constexpr int KMAX_ALL = 256;        // largest k any kernel accepts
+constexpr int KLUT = KMAX_ALL + 2;   // lut index: 0 = k < 1, 1..256 = k, 257 = k > 256
+constexpr int MAXBIN = 272;
+constexpr int BIN_COUNT = 0, BIN_START = MAXBIN, BIN_FILL = 2 * MAXBIN;
+struct BinLut {
+  uint16_t bin[KLUT];
+};
+__host__ __device__ inline int lut_index(int64_t k) { return k < 1 ? 0 : k > KMAX_ALL ? KMAX_ALL + 1 : (int)k; }
+
 struct SolveArgs {
   const float *G;          // n x n, row-major, leading dimension ld (floats)
   int64_t n, ld;
   const int32_t *idx;      // concatenated index sets
   const int64_t *offsets;  // nsub+1 offsets into idx, or nullptr (uniform k)
   int kuni;                // subset size when offsets == nullptr
-  int64_t nslots;          // work items in this launch
+  int kcap;                // largest valid k (the T stride); larger subsets get BADK
+  int64_t nslots;          // work items in this launch (ignored when bins != nullptr)
   const int32_t *order;    // slot -> subset id, or nullptr (identity)
+  const int32_t *bins;     // device bin table: when set, the launch drains bin `bin`
+  int bin;                 //   (slots 0..count-1 of order + start) with a grid-stride loop
   uint32_t *status;        // per-subset status word
   float2 *tri;             // T of subset s at tri + s*tstride, (alpha_i, beta_i)
   int tstride;
@@ -29,6 +44,7 @@ struct SolveArgs {
 };
 
 struct BisectArgs {
+  const int32_t *order;    // slot -> subset id, or nullptr (identity)
   const float2 *tri;
   int tstride;
   const double4 *bounds;
@@ -40,17 +56,50 @@ struct BisectArgs {
   double *eig, *det;
 };
 
-// workspace of a solve: T entries, then (256-byte aligned) per-subset bounds
+// workspace of a solve, each part 256-byte aligned: T entries, per-subset
+// bounds, the bin table and the order array
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 inline size_t tridiag_bytes(int64_t nsub, int kstride) {
-  return ((size_t)nsub * (size_t)kstride * sizeof(float2) + 255) & ~(size_t)255;
+  return align256((size_t)nsub * (size_t)kstride * sizeof(float2));
 }
+inline size_t bounds_bytes(int64_t nsub) { return align256((size_t)nsub * sizeof(double4)); }
+inline size_t bintable_bytes() { return align256(3 * MAXBIN * sizeof(int32_t)); }
 inline size_t workspace_bytes(int64_t nsub, int kstride) {
-  return tridiag_bytes(nsub, kstride) + (size_t)nsub * sizeof(double4);
+  return tridiag_bytes(nsub, kstride) + bounds_bytes(nsub) + bintable_bytes() +
+         align256((size_t)nsub * sizeof(int32_t));
+}
+struct WorkspaceParts {
+  float2 *tri;
+  double4 *bounds;
+  int32_t *table;
+  int32_t *order;
+};
+inline WorkspaceParts split_workspace(void *p, int64_t nsub, int kstride) {
+  char *c = (char *)p;
+  WorkspaceParts w;
+  w.tri = (float2 *)c;
+  c += tridiag_bytes(nsub, kstride);
+  w.bounds = (double4 *)c;
+  c += bounds_bytes(nsub);
+  w.table = (int32_t *)c;
+  c += bintable_bytes();
+  w.order = (int32_t *)c;
+  return w;
 }
 
-// lanes per subset (8, 16, 32, 64) x LDS row padding (0, 1, 4) x mode
-hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStream_t s);
+// lanes per subset (8, 16, 32, 64, 128, 256) x LDS row padding (0, 1, 4) x mode.
+// With a.bins == nullptr the grid covers a.nslots slots; otherwise the launch
+// is persistent (grid = resident workgroups, capped by `max_slots`) and drains
+// bin a.bin of the table.
+// f64 selects FP64 Gram-Schmidt coefficients (lane groups of 128 and 256).
+hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStream_t s,
+                        int64_t max_slots = 0, bool f64 = false);
 bool solve_variant_exists(int group, int pad);
+
+// bin the nsub subsets (sizes from offsets, or kuni) by lut into table/order:
+// clear, count, scan, scatter - four stream-ordered launches, no host sync
+hipError_t launch_binning(const int64_t *offsets, int kuni, int64_t nsub, const BinLut &lut,
+                          int nbins, int32_t *table, int32_t *order, hipStream_t s);
 // mode 0: lambda_min/lambda_max/delta; mode 1: every eigenvalue + det (uniform k)
 hipError_t launch_bisect(int mode, const BisectArgs &a, hipStream_t s);
 

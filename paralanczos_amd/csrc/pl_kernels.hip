@@ -34,6 +34,10 @@
 // one wavefront, so groups finish independently.
 #include <float.h>
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include <math.h>
 
 #include "pl_common.hpp"
@@ -515,26 +519,12 @@ __device__ __forceinline__ void store_tridiag(float2 *tri, double4 *bounds, int 
 // waves per SIMD each lane-group size is register-budgeted for (VGPRs <= 512/n)
 template <int GROUP>
 constexpr int waves_per_simd() {
-  return GROUP == 8 ? 8 : GROUP == 16 ? 7 : GROUP == 32 ? 4 : 2;
+  return GROUP == 8 ? 7 : GROUP == 16 ? 6 : GROUP == 32 ? 4 : 2;
 }
 
 template <int GROUP, int PAD, int MODE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(waves_per_simd<GROUP>())))
-void solve_kernel(SolveArgs a) {
-  constexpr int SPB = BLOCK / GROUP;
+__device__ __forceinline__ void solve_subset(const SolveArgs &a, float *R, int gl, int64_t s) {
   constexpr int STRIDE = GROUP + PAD;
-  constexpr int REGION = GROUP * STRIDE;
-  static_assert(REGION >= 5 * GROUP, "LDS region too small for the vector buffers");
-  __shared__ __attribute__((aligned(16))) float lds[SPB * REGION];
-
-  const int tid = threadIdx.x;
-  const int gl = tid % GROUP;
-  const int gib = tid / GROUP;
-  float *R = lds + gib * REGION;
-
-  const int64_t slot = (int64_t)blockIdx.x * SPB + gib;
-  if (slot >= a.nslots) return;  // whole group leaves; nothing below syncs the block
-  const int64_t s = a.order ? (int64_t)a.order[slot] : slot;
   int k;
   int64_t off;
   if (a.offsets) {
@@ -544,7 +534,7 @@ void solve_kernel(SolveArgs a) {
     k = a.kuni;
     off = s * (int64_t)k;
   }
-  uint32_t st = (k < 1 || k > GROUP) ? ST_BADK : 0u;
+  uint32_t st = (k < 1 || k > GROUP || k > a.kcap) ? ST_BADK : 0u;
   int my = 0;
   if (!st && gl < k) {
     my = a.idx[off + gl];
@@ -619,6 +609,35 @@ void solve_kernel(SolveArgs a) {
   }
 }
 
+template <int GROUP, int PAD, int MODE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(waves_per_simd<GROUP>())))
+void solve_kernel(SolveArgs a) {
+  constexpr int SPB = BLOCK / GROUP;
+  constexpr int STRIDE = GROUP + PAD;
+  constexpr int REGION = GROUP * STRIDE;
+  static_assert(REGION >= 5 * GROUP, "LDS region too small for the vector buffers");
+  __shared__ __attribute__((aligned(16))) float lds[SPB * REGION];
+
+  const int tid = threadIdx.x;
+  const int gl = tid % GROUP;
+  const int gib = tid / GROUP;
+  float *R = lds + gib * REGION;
+
+  // slots of this launch: all of them, or the run of one bin (persistent grid);
+  // lane groups advance independently, nothing below syncs the block
+  int64_t cnt = a.nslots;
+  const int32_t *ord = a.order;
+  if (a.bins) {
+    cnt = a.bins[BIN_COUNT + a.bin];
+    ord = a.order + a.bins[BIN_START + a.bin];
+  }
+  for (int64_t slot = (int64_t)blockIdx.x * SPB + gib; slot < cnt;
+       slot += (int64_t)gridDim.x * SPB) {
+    solve_subset<GROUP, PAD, MODE>(a, R, gl, ord ? (int64_t)ord[slot] : slot);
+    wave_sync();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Wide subsets (64 < k <= 128): one 128-lane workgroup (two wavefronts) per
 // subset. Lane i owns row i of G_S in VGPRs; the padded LDS tile that receives
@@ -629,8 +648,13 @@ void solve_kernel(SolveArgs a) {
 // ---------------------------------------------------------------------------
 constexpr int WIDE = 128;
 
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <typename T>
 struct WideShared {
-  float lv[WIDE], lw[WIDE], lh[WIDE];
+  float lv[WIDE], lw[WIDE];
+  T lh[WIDE];  // Gram-Schmidt coefficients (FP32, or FP64 coefficients)
   float red[8];
   double redd[8];
   int ired[4];
@@ -638,7 +662,8 @@ struct WideShared {
 
 // compensated dot over the workgroup: per-wave Dot2, then the two wave
 // partials combined in wave order (every lane gets identical bits)
-__device__ __forceinline__ float wide_dot2(float x, float y, WideShared &sh) {
+template <typename T>
+__device__ __forceinline__ float wide_dot2(float x, float y, WideShared<T> &sh) {
   float p, e;
   g_dot2_pe<64>(x, y, p, e);
   const int w = threadIdx.x >> 6;
@@ -653,7 +678,8 @@ __device__ __forceinline__ float wide_dot2(float x, float y, WideShared &sh) {
   return p0 + e0;
 }
 
-__device__ __forceinline__ float wide_sum(float x, WideShared &sh) {
+template <typename T>
+__device__ __forceinline__ float wide_sum(float x, WideShared<T> &sh) {
   x = g_sum_dpp<64>(x);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh.red[4 + (threadIdx.x >> 6)] = x;
@@ -661,7 +687,8 @@ __device__ __forceinline__ float wide_sum(float x, WideShared &sh) {
   return sh.red[4] + sh.red[5];
 }
 
-__device__ __forceinline__ double wide_min(double x, WideShared &sh) {
+template <typename T>
+__device__ __forceinline__ double wide_min(double x, WideShared<T> &sh) {
   x = g_min<64>(x);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh.redd[threadIdx.x >> 6] = x;
@@ -669,7 +696,8 @@ __device__ __forceinline__ double wide_min(double x, WideShared &sh) {
   return fmin(sh.redd[0], sh.redd[1]);
 }
 
-__device__ __forceinline__ double wide_max(double x, WideShared &sh) {
+template <typename T>
+__device__ __forceinline__ double wide_max(double x, WideShared<T> &sh) {
   x = g_max<64>(x);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh.redd[2 + (threadIdx.x >> 6)] = x;
@@ -694,28 +722,28 @@ __device__ __forceinline__ float lds_dot(const float *L, const float (&x)[N]) {
 }
 
 // Gram-Schmidt pass against the LDS basis: lane l forms h_l = v_l . w from
-// column l of V, lane i subtracts sum_l V[i][l] h_l from row i
-template <int S>
-__device__ __forceinline__ float wide_cgs(float w, const float *V, WideShared &sh) {
+// column l of V, lane i subtracts sum_l V[i][l] h_l from row i; coefficients
+// and the update sum are accumulated in T
+template <int S, typename T>
+__device__ __forceinline__ float wide_cgs(float w, const float *V, WideShared<T> &sh) {
   const int gl = threadIdx.x;
   sh.lw[gl] = w;
   __syncthreads();
-  float acc0 = 0.f, acc1 = 0.f;
+  T acc0 = 0, acc1 = 0;
 #pragma unroll 8
   for (int q = 0; q < WIDE / 4; ++q) {
     const float4 t = reinterpret_cast<const float4 *>(sh.lw)[q];
-    acc0 = __builtin_fmaf(V[(4 * q + 0) * S + gl], t.x, acc0);
-    acc1 = __builtin_fmaf(V[(4 * q + 1) * S + gl], t.y, acc1);
-    acc0 = __builtin_fmaf(V[(4 * q + 2) * S + gl], t.z, acc0);
-    acc1 = __builtin_fmaf(V[(4 * q + 3) * S + gl], t.w, acc1);
+    acc0 = fmaT((T)V[(4 * q + 0) * S + gl], (T)t.x, acc0);
+    acc1 = fmaT((T)V[(4 * q + 1) * S + gl], (T)t.y, acc1);
+    acc0 = fmaT((T)V[(4 * q + 2) * S + gl], (T)t.z, acc0);
+    acc1 = fmaT((T)V[(4 * q + 3) * S + gl], (T)t.w, acc1);
   }
   sh.lh[gl] = acc0 + acc1;
   __syncthreads();
-  acc0 = 0.f;
-  acc1 = 0.f;
+  acc0 = 0;
+  acc1 = 0;
 #pragma unroll 8
   for (int q = 0; q < WIDE / 4; ++q) {
-    const float4 t = reinterpret_cast<const float4 *>(sh.lh)[q];
     float4 r;
     if constexpr (S % 4 == 0) {
       r = reinterpret_cast<const float4 *>(V + gl * S)[q];
@@ -723,23 +751,19 @@ __device__ __forceinline__ float wide_cgs(float w, const float *V, WideShared &s
       r = make_float4(V[gl * S + 4 * q], V[gl * S + 4 * q + 1], V[gl * S + 4 * q + 2],
                       V[gl * S + 4 * q + 3]);
     }
-    acc0 = __builtin_fmaf(r.x, t.x, acc0);
-    acc1 = __builtin_fmaf(r.y, t.y, acc1);
-    acc0 = __builtin_fmaf(r.z, t.z, acc0);
-    acc1 = __builtin_fmaf(r.w, t.w, acc1);
+    acc0 = fmaT((T)r.x, sh.lh[4 * q + 0], acc0);
+    acc1 = fmaT((T)r.y, sh.lh[4 * q + 1], acc1);
+    acc0 = fmaT((T)r.z, sh.lh[4 * q + 2], acc0);
+    acc1 = fmaT((T)r.w, sh.lh[4 * q + 3], acc1);
   }
-  return w - (acc0 + acc1);
+  return (float)((T)w - (acc0 + acc1));
 }
 
-template <int PAD, int MODE>
-__global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
+template <int PAD, int MODE, typename T>
+__device__ __forceinline__ void solve_wide_subset(const SolveArgs &a, float *V, WideShared<T> &sh,
+                                                  int64_t s) {
   constexpr int S = WIDE + PAD;
-  __shared__ __attribute__((aligned(16))) float V[WIDE * S];
-  __shared__ __attribute__((aligned(16))) WideShared sh;
   const int gl = threadIdx.x;
-  const int64_t slot = blockIdx.x;
-  if (slot >= a.nslots) return;  // uniform over the workgroup
-  const int64_t s = a.order ? (int64_t)a.order[slot] : slot;
   int k;
   int64_t off;
   if (a.offsets) {
@@ -749,13 +773,14 @@ __global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
     k = a.kuni;
     off = s * (int64_t)k;
   }
-  uint32_t st = (k < 1 || k > WIDE) ? ST_BADK : 0u;
+  const bool badk = k < 1 || k > WIDE || k > a.kcap;
+  uint32_t st = badk ? ST_BADK : 0u;
   int my = 0;
   if (!st && gl < k) {
     my = a.idx[off + gl];
     if (my < 0 || (int64_t)my >= a.n) st = ST_BADINDEX;
   }
-  if (__syncthreads_or(st != 0)) st |= (k < 1 || k > WIDE) ? ST_BADK : ST_BADINDEX;
+  if (__syncthreads_or(st != 0)) st |= badk ? ST_BADK : ST_BADINDEX;
   int *lidx = reinterpret_cast<int *>(sh.lv);
   lidx[gl] = my;
   __syncthreads();
@@ -826,11 +851,11 @@ __global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
         amax = fmaxf(amax, fabsf(al));
         w = __builtin_fmaf(-al, vi, __builtin_fmaf(-bj, vp, w));
         const float pre = wide_sum(w * w, sh);
-        w = wide_cgs<S>(w, V, sh);
+        w = wide_cgs<S, T>(w, V, sh);
         if (j == k - 1) break;
         float b2 = wide_dot2(w, w, sh);
         if (b2 < 0.5f * pre) {  // DGKS: w was mostly in span(V); orthogonalise again
-          w = wide_cgs<S>(w, V, sh);
+          w = wide_cgs<S, T>(w, V, sh);
           b2 = wide_dot2(w, w, sh);
         }
         const float b = sqrtf(b2);
@@ -857,8 +882,8 @@ __global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
           __syncthreads();
           const int bi = (sh.red[1] > sh.red[0]) ? sh.ired[1] : sh.ired[0];
           w = (gl == bi) ? 1.f : 0.f;
-          w = wide_cgs<S>(w, V, sh);
-          w = wide_cgs<S>(w, V, sh);
+          w = wide_cgs<S, T>(w, V, sh);
+          w = wide_cgs<S, T>(w, V, sh);
           scale = 1.0f / sqrtf(wide_dot2(w, w, sh));
         }
         vi = w * scale;
@@ -892,6 +917,451 @@ __global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
   }
 }
 
+template <int PAD, int MODE, typename T = float>
+__global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
+  __shared__ __attribute__((aligned(16))) float V[WIDE * (WIDE + PAD)];
+  __shared__ __attribute__((aligned(16))) WideShared<T> sh;
+  int64_t cnt = a.nslots;
+  const int32_t *ord = a.order;
+  if (a.bins) {
+    cnt = a.bins[BIN_COUNT + a.bin];
+    ord = a.order + a.bins[BIN_START + a.bin];
+  }
+  // the slot is uniform over the workgroup, so every barrier inside is too
+  for (int64_t slot = blockIdx.x; slot < cnt; slot += gridDim.x) {
+    solve_wide_subset<PAD, MODE, T>(a, V, sh, ord ? (int64_t)ord[slot] : slot);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Large subsets (128 < k <= 256): one 512-lane workgroup (eight wavefronts,
+// one subset per CU). Lane t = (row i = t % 256, half h = t / 256) holds
+// G_S[i][128h .. 128h+127] in VGPRs. FP32 G_S and an FP32 basis do not both
+// fit one CU's LDS, so the Lanczos basis V is split: columns 0..K2L-1 live in
+// LDS (row-major, stride K2S), columns K2L..255 in VGPRs, column K2L + 2m + h
+// of row i as vr[m] on lane (i, h). Gram-Schmidt coefficients of the LDS
+// columns are column dot products (lane t forms column t % 256's over the
+// rows of half t / 256); those of the register columns come from a wavefront
+// reduce-scatter of the lanes' products (wave_rs16) and a sum over the four
+// wavefronts of each half in LDS. Coefficients, and the update they feed, are
+// accumulated in T: float, or double (FP64 reorthogonalisation coefficients).
+// ---------------------------------------------------------------------------
+constexpr int K2 = 256;                      // subset capacity
+constexpr int K2T = 512;                     // threads per workgroup
+constexpr int K2L = 140;                     // basis columns in LDS
+constexpr int K2S = 140;                     // LDS row stride: 16-byte rows, stride/4 odd
+constexpr int K2R = (K2 - K2L) / 2;          // register basis columns per lane
+constexpr int K2C = 16;                      // reduce-scatter chunk
+constexpr int K2NC = (K2R + K2C - 1) / K2C;  // chunks
+constexpr int K2H = 72;                      // LDS columns updated by half 0 (rest: half 1)
+constexpr int DPP_ROW_ROR8 = 0x128;          // lane i <- lane (i + 8) mod 16 within a row
+
+template <typename T>
+struct K2Shared {
+  float lv[K2];      // v_j broadcast
+  float lw[K2];      // w broadcast (also the index staging of the gather)
+  float pw[2][K2];   // matvec partials of the two halves
+  T part[2][K2];     // Gram-Schmidt update partials of the two halves
+  T lh[K2];          // Gram-Schmidt coefficients
+  T cpart[K2];       // LDS-column dot partials of half 1
+  T rpart[2][4][K2NC * K2C];  // register-column partials per half and wavefront
+  float red[16];
+  double redd[16];
+  int ired[8];
+};
+static_assert(K2 * K2S * sizeof(float) + sizeof(K2Shared<double>) <= 163840, "LDS budget");
+static_assert(K2S % 4 == 0 && (K2S / 4) % 2 == 1 && K2H % 4 == 0 && (K2L - K2H) % 4 == 0,
+              "row layout");
+
+template <typename T>
+__device__ __forceinline__ T swap_add(T a, T b, bool wide32);
+template <>
+__device__ __forceinline__ float swap_add<float>(float a, float b, bool wide32) {
+  // v_permlane32_swap (lanes 32..63 of a <-> lanes 0..31 of b) or
+  // v_permlane16_swap (odd rows of a <-> even rows of b): afterwards the lane
+  // holds its own and its partner's value of the element its half keeps
+  const unsigned ua = __builtin_bit_cast(unsigned, a), ub = __builtin_bit_cast(unsigned, b);
+  const auto r = wide32 ? __builtin_amdgcn_permlane32_swap(ua, ub, false, false)
+                        : __builtin_amdgcn_permlane16_swap(ua, ub, false, false);
+  return __builtin_bit_cast(float, (unsigned)r[0]) + __builtin_bit_cast(float, (unsigned)r[1]);
+}
+template <>
+__device__ __forceinline__ double swap_add<double>(double a, double b, bool wide32) {
+  const unsigned long long ua = __builtin_bit_cast(unsigned long long, a);
+  const unsigned long long ub = __builtin_bit_cast(unsigned long long, b);
+  const unsigned alo = (unsigned)ua, ahi = (unsigned)(ua >> 32);
+  const unsigned blo = (unsigned)ub, bhi = (unsigned)(ub >> 32);
+  const auto lo = wide32 ? __builtin_amdgcn_permlane32_swap(alo, blo, false, false)
+                         : __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+  const auto hi = wide32 ? __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false)
+                         : __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+  const double x0 = __builtin_bit_cast(
+      double, ((unsigned long long)(unsigned)hi[0] << 32) | (unsigned)lo[0]);
+  const double x1 = __builtin_bit_cast(
+      double, ((unsigned long long)(unsigned)hi[1] << 32) | (unsigned)lo[1]);
+  return x0 + x1;
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xf, 0xf, true);
+  const unsigned hi =
+      (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Reduce-scatter of 16 values over the 64 lanes of a wavefront: lane q
+// returns sum over all lanes of x[q >> 2] (lanes 4e..4e+3 all hold element
+// e). Partners by stage: lane ^ 32, ^ 16 (permlane swaps), ^ 8 (row_ror:8),
+// ^ 7 (row_half_mirror), then ^ 2 and ^ 1 (quad_perm) as an all-reduce.
+template <typename T>
+__device__ __forceinline__ T wave_rs16(const T (&x)[16], int q) {
+  T y[8], z[4], v[2];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) y[u] = swap_add<T>(x[u], x[u + 8], true);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) z[u] = swap_add<T>(y[u], y[u + 4], false);
+  const bool b3 = (q >> 3) & 1, b2 = (q >> 2) & 1;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const T send = b3 ? z[u] : z[u + 2];
+    const T keep = b3 ? z[u + 2] : z[u];
+    v[u] = keep + dpp<DPP_ROW_ROR8>(send);
+  }
+  const T send = b2 ? v[0] : v[1];
+  T r = (b2 ? v[1] : v[0]) + dpp<DPP_ROW_HMIRROR>(send);
+  r += dpp<DPP_QUAD_2301>(r);
+  r += dpp<DPP_QUAD_1032>(r);
+  return r;
+}
+
+// compensated dot over the 256 rows (the four wavefronts of half 0; half 1
+// holds the same rows and takes the same result)
+template <typename T>
+__device__ __forceinline__ float k2_dot2(float x, float y, K2Shared<T> &sh) {
+  float p, e;
+  g_dot2_pe<64>(x, y, p, e);
+  const int t = threadIdx.x;
+  __syncthreads();
+  if (t < K2 && (t & 63) == 0) {
+    sh.red[2 * (t >> 6)] = p;
+    sh.red[2 * (t >> 6) + 1] = e;
+  }
+  __syncthreads();
+  float p0 = sh.red[0], e0 = sh.red[1];
+  comp_step(p0, e0, sh.red[2], sh.red[3]);
+  comp_step(p0, e0, sh.red[4], sh.red[5]);
+  comp_step(p0, e0, sh.red[6], sh.red[7]);
+  return p0 + e0;
+}
+
+template <typename T>
+__device__ __forceinline__ float k2_sum(float x, K2Shared<T> &sh) {
+  x = g_sum_dpp<64>(x);
+  const int t = threadIdx.x;
+  __syncthreads();
+  if (t < K2 && (t & 63) == 0) sh.red[8 + (t >> 6)] = x;
+  __syncthreads();
+  return ((sh.red[8] + sh.red[9]) + sh.red[10]) + sh.red[11];
+}
+
+template <bool MAX, typename T>
+__device__ __forceinline__ double k2_minmax(double x, int slot, K2Shared<T> &sh) {
+  x = MAX ? g_max<64>(x) : g_min<64>(x);
+  const int t = threadIdx.x;
+  __syncthreads();
+  if (t < K2 && (t & 63) == 0) sh.redd[4 * slot + (t >> 6)] = x;
+  __syncthreads();
+  const double *r = sh.redd + 4 * slot;
+  return MAX ? fmax(fmax(r[0], r[1]), fmax(r[2], r[3]))
+             : fmin(fmin(r[0], r[1]), fmin(r[2], r[3]));
+}
+
+// sum_c g[c] * L[c] over 128 entries, L a 16-byte aligned LDS vector read by
+// broadcast; four accumulators by index mod 4
+__device__ __forceinline__ float k2_rowdot(const float (&g)[128], const float *L) {
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+#pragma unroll
+  for (int q = 0; q < 32; ++q) {
+    const float4 t = reinterpret_cast<const float4 *>(L)[q];
+    a0 = __builtin_fmaf(g[4 * q + 0], t.x, a0);
+    a1 = __builtin_fmaf(g[4 * q + 1], t.y, a1);
+    a2 = __builtin_fmaf(g[4 * q + 2], t.z, a2);
+    a3 = __builtin_fmaf(g[4 * q + 3], t.w, a3);
+  }
+  return (a0 + a1) + (a2 + a3);
+}
+
+
+// One classical Gram-Schmidt pass of w (row i's entry, identical on both
+// halves) against basis columns 0..j; returns the new entry.
+template <typename T>
+__device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
+                                        const float (&vr)[K2R], K2Shared<T> &sh) {
+  const int t = threadIdx.x, i = t & (K2 - 1), h = t >> 8, wv = (t >> 6) & 3, q = t & 63;
+  if (h == 0) sh.lw[i] = w;
+  __syncthreads();
+  // (1) LDS columns: column i over the rows of half h (rows >= k are zero)
+  const int ncl = min(j + 1, K2L);
+  T colp = 0;
+  if (i < ncl) {
+    const int r0 = 128 * h;
+    const int rn = (min(128, max(0, k - r0)) + 7) & ~7;
+    T a0 = 0, a1 = 0;
+    for (int r = 0; r < rn; r += 8) {
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        a0 = fmaT((T)VL[(r0 + r + u) * K2S + i], (T)sh.lw[r0 + r + u], a0);
+        a1 = fmaT((T)VL[(r0 + r + u + 1) * K2S + i], (T)sh.lw[r0 + r + u + 1], a1);
+      }
+    }
+    colp = a0 + a1;
+    if (h == 1) sh.cpart[i] = colp;
+  }
+  // (2) register columns K2L + 2m + h, m < nm, by reduce-scatter per chunk
+  const int nm = (max(0, j + 1 - K2L) + 1) >> 1;
+#pragma unroll
+  for (int c = 0; c < K2NC; ++c) {
+    if (K2C * c < nm) {
+      T x[K2C];
+#pragma unroll
+      for (int u = 0; u < K2C; ++u) {
+        const int m = K2C * c + u;
+        x[u] = (m < K2R) ? (T)vr[m < K2R ? m : 0] * (T)w : (T)0;
+      }
+      const T r = wave_rs16<T>(x, q);
+      if ((q & 3) == 0) sh.rpart[h][wv][K2C * c + (q >> 2)] = r;
+    }
+  }
+  __syncthreads();
+  // (3) the coefficients, lane t < 256 for column t (zero beyond j)
+  if (h == 0) {
+    T hc = 0;
+    if (i < ncl) {
+      hc = colp + sh.cpart[i];
+    } else if (i >= K2L && i <= j) {
+      const int m = (i - K2L) >> 1, hh = (i - K2L) & 1;
+      hc = ((sh.rpart[hh][0][m] + sh.rpart[hh][1][m]) + sh.rpart[hh][2][m]) + sh.rpart[hh][3][m];
+    }
+    sh.lh[i] = hc;
+  }
+  __syncthreads();
+  // (4) update partials: LDS columns [0, K2H) by half 0 and [K2H, K2L) by
+  // half 1, register columns by their owner half
+  T u0 = 0, u1 = 0;
+  {
+    const int c0 = h == 0 ? 0 : K2H;
+    const int cn = h == 0 ? min(ncl, K2H) : max(0, ncl - K2H);
+    const float *row = VL + i * K2S + c0;
+    const T *hl = sh.lh + c0;
+    for (int c = 0; c < cn; c += 4) {
+      const float4 r = *reinterpret_cast<const float4 *>(row + c);
+      u0 = fmaT((T)r.x, hl[c + 0], u0);
+      u1 = fmaT((T)r.y, hl[c + 1], u1);
+      u0 = fmaT((T)r.z, hl[c + 2], u0);
+      u1 = fmaT((T)r.w, hl[c + 3], u1);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < K2R; ++m)
+    if (m < nm) u0 = fmaT((T)vr[m], sh.lh[K2L + 2 * m + h], u0);
+  sh.part[h][i] = u0 + u1;
+  __syncthreads();
+  return (float)((T)w - (sh.part[0][i] + sh.part[1][i]));
+}
+
+template <int MODE, typename T>
+__device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Shared<T> &sh,
+                                          int64_t s) {
+  const int t = threadIdx.x, i = t & (K2 - 1), h = t >> 8, q = t & 63;
+  int k;
+  int64_t off;
+  if (a.offsets) {
+    off = a.offsets[s];
+    k = (int)(a.offsets[s + 1] - off);
+  } else {
+    k = a.kuni;
+    off = s * (int64_t)k;
+  }
+  const bool badk = k < 1 || k > K2 || k > a.kcap;
+  uint32_t st = badk ? ST_BADK : 0u;
+  int my = 0;
+  if (!st && i < k) {
+    my = a.idx[off + i];
+    if (my < 0 || (int64_t)my >= a.n) st = ST_BADINDEX;
+  }
+  if (__syncthreads_or(st != 0)) st |= badk ? ST_BADK : ST_BADINDEX;
+  int *lidx = reinterpret_cast<int *>(sh.lw);
+  if (h == 0) lidx[i] = my;
+  __syncthreads();
+
+  // ---- (a) gather: lane (i, h) loads G_S[i][128h + c] directly ----
+  float g[128];
+  bool fin = true;
+#pragma unroll
+  for (int c = 0; c < 128; ++c) {
+    const int col = 128 * h + c;
+    float x = 0.f;
+    if (st == 0 && i < k && col < k) {
+      const int ic = lidx[col];
+      const int64_t lo = min(ic, my), hi = max(ic, my);
+      x = a.G[lo * a.ld + hi];
+    }
+    g[c] = x;
+    fin = fin && isfinite(x);
+  }
+  if (__syncthreads_or(!fin)) st |= ST_NONFINITE;
+
+  if constexpr (MODE == MODE_DUMP) {
+    if (i < k && st == 0) {
+      float *o = a.gs_out + s * (int64_t)k * k + (int64_t)i * k + 128 * h;
+#pragma unroll
+      for (int c = 0; c < 128; ++c)
+        if (128 * h + c < k) o[c] = g[c];
+    }
+    return;
+  } else if constexpr (MODE == MODE_CHECKSUM) {
+    float acc = 0.f;
+#pragma unroll
+    for (int c = 0; c < 128; ++c) acc += g[c];
+    acc = g_sum<64>(acc);
+    if (q == 0) sh.red[t >> 6] = acc;
+    __syncthreads();
+    if (t == 0) {
+      float sum = 0.f;
+      for (int w = 0; w < 8; ++w) sum += sh.red[w];
+      a.gs_out[s] = st ? NAN : sum;
+    }
+    return;
+  } else {
+    constexpr uint32_t FATAL = ST_BADK | ST_BADINDEX | ST_NONFINITE;
+    if (!(st & FATAL)) {
+      // ---- (b) Lanczos recurrence, ParaLanczos.cu:1285-1374 with the build's
+      // substitutions (ones/sqrt(k) start, full reorthogonalisation, restart) ----
+      float vr[K2R];
+#pragma unroll
+      for (int m = 0; m < K2R; ++m) vr[m] = 0.f;
+      for (int e = t; e < K2 * K2S; e += K2T) VL[e] = 0.f;
+      float vi = (i < k) ? (1.0f / sqrtf((float)k)) : 0.f;
+      __syncthreads();
+      if (h == 0) VL[i * K2S] = vi;
+      float vp = 0.f, bj = 0.f, amax = 0.f, bmax = 0.f, my_a = 0.f, my_b = 0.f;
+      for (int j = 0; j < k; ++j) {
+        if (h == 0) sh.lv[i] = vi;
+        __syncthreads();
+        sh.pw[h][i] = k2_rowdot(g, sh.lv + 128 * h);
+        __syncthreads();
+        float w = sh.pw[0][i] + sh.pw[1][i];
+        const float al = k2_dot2(vi, w, sh);
+        my_a = (i == j) ? al : my_a;
+        amax = fmaxf(amax, fabsf(al));
+        w = __builtin_fmaf(-al, vi, __builtin_fmaf(-bj, vp, w));
+        const float pre = k2_sum(w * w, sh);
+        w = k2_cgs<T>(w, j, k, VL, vr, sh);
+        if (j == k - 1) break;
+        float b2 = k2_dot2(w, w, sh);
+        if (b2 < 0.5f * pre) {  // DGKS: w was mostly in span(V); orthogonalise again
+          w = k2_cgs<T>(w, j, k, VL, vr, sh);
+          b2 = k2_dot2(w, w, sh);
+        }
+        const float b = sqrtf(b2);
+        vp = vi;
+        float scale;
+        if (b > TAU * (amax + bmax)) {
+          bj = b;
+          bmax = fmaxf(bmax, b);
+          scale = 1.0f / b;
+        } else {
+          // restart: e_i for the first i of largest residual 1 - ||V[i][:]||^2,
+          // orthogonalised twice against the basis; beta_{j+1} = 0
+          st |= ST_BREAKDOWN;
+          bj = 0.f;
+          float rs = 0.f;
+          if (h == 0)
+            for (int l = 0; l < K2L; ++l) rs = __builtin_fmaf(VL[i * K2S + l], VL[i * K2S + l], rs);
+#pragma unroll
+          for (int m = 0; m < K2R; ++m) rs = __builtin_fmaf(vr[m], vr[m], rs);
+          __syncthreads();
+          sh.pw[h][i] = rs;
+          __syncthreads();
+          const float res = 1.f - (sh.pw[0][i] + sh.pw[1][i]);
+          const int wb = restart_lane<64>(res, i, k);
+          const float wvv = __shfl(i < k ? res : -1.f, wb & 63, 64);
+          __syncthreads();
+          if (t < K2 && q == 0) {
+            sh.red[t >> 6] = wvv;
+            sh.ired[t >> 6] = wb;
+          }
+          __syncthreads();
+          int bi = sh.ired[0];
+          float bv = sh.red[0];
+          for (int w2 = 1; w2 < 4; ++w2)
+            if (sh.red[w2] > bv) {
+              bv = sh.red[w2];
+              bi = sh.ired[w2];
+            }
+          w = (i == bi) ? 1.f : 0.f;
+          w = k2_cgs<T>(w, j, k, VL, vr, sh);
+          w = k2_cgs<T>(w, j, k, VL, vr, sh);
+          scale = 1.0f / sqrtf(k2_dot2(w, w, sh));
+        }
+        vi = w * scale;
+        // basis column j + 1: LDS, or register slot (j + 1 - K2L) / 2 of one half
+        const int c = j + 1;
+        if (c < K2L) {
+          if (h == 0) VL[i * K2S + c] = vi;
+        } else {
+          const int m = (c - K2L) >> 1;
+          const bool mine = ((c - K2L) & 1) == h;
+#pragma unroll
+          for (int r = 0; r < K2R; ++r) vr[r] = (mine && r == m) ? vi : vr[r];
+        }
+        my_b = (i == j + 1) ? bj : my_b;
+      }
+      const bool finite = (i >= k) || (isfinite(my_a) && isfinite(my_b));
+      if (__syncthreads_or(!finite)) st |= ST_NONFINITE;
+      if (!(st & FATAL)) {
+        float2 *tri = a.tri + s * (int64_t)a.tstride;
+        sh.pw[0][i] = my_b;  // both halves write the same value
+        __syncthreads();
+        const double al = my_a, b = my_b, bn = (i + 1 < K2) ? (double)sh.pw[0][i + 1] : 0.0;
+        double glo = INFINITY, ghi = -INFINITY, amin = INFINITY, amx = -INFINITY;
+        if (i < k) {
+          if (h == 0) tri[i] = make_float2(my_a, i > 0 ? my_b : 0.f);
+          const double r = (i > 0 ? fabs(b) : 0.0) + (i + 1 < k ? fabs(bn) : 0.0);
+          glo = al - r;
+          ghi = al + r;
+          amin = amx = al;
+        }
+        glo = k2_minmax<false>(glo, 0, sh);
+        ghi = k2_minmax<true>(ghi, 1, sh);
+        amin = k2_minmax<false>(amin, 2, sh);
+        amx = k2_minmax<true>(amx, 3, sh);
+        if (t == 0) a.bounds[s] = make_double4(glo, ghi, amin, amx);
+      }
+    }
+    if (t == 0) a.status[s] = st;
+  }
+}
+
+template <int MODE, typename T>
+__global__ __launch_bounds__(K2T) void solve_k256_kernel(SolveArgs a) {
+  __shared__ __attribute__((aligned(16))) float VL[K2 * K2S];
+  __shared__ __attribute__((aligned(16))) K2Shared<T> sh;
+  int64_t cnt = a.nslots;
+  const int32_t *ord = a.order;
+  if (a.bins) {
+    cnt = a.bins[BIN_COUNT + a.bin];
+    ord = a.order + a.bins[BIN_START + a.bin];
+  }
+  for (int64_t slot = blockIdx.x; slot < cnt; slot += gridDim.x) {
+    k2_subset<MODE, T>(a, VL, sh, ord ? (int64_t)ord[slot] : slot);
+    __syncthreads();
+  }
+}
+
 using SolveFn = void (*)(SolveArgs);
 
 template <int GROUP, int PAD>
@@ -900,12 +1370,22 @@ struct Variant {
                                      solve_kernel<GROUP, PAD, 2>};
 };
 
-static SolveFn pick(int group, int pad, int mode) {
+static SolveFn pick(int group, int pad, int mode, bool f64) {
+  if (group == K2) {
+    if (pad != 4) return nullptr;
+    static constexpr SolveFn k32[3] = {solve_k256_kernel<0, float>, solve_k256_kernel<1, float>,
+                                       solve_k256_kernel<2, float>};
+    return (f64 && mode == MODE_TRIDIAG) ? solve_k256_kernel<0, double> : k32[mode];
+  }
   if (group == WIDE) {
     static constexpr SolveFn w1[3] = {solve_wide_kernel<1, 0>, solve_wide_kernel<1, 1>,
                                       solve_wide_kernel<1, 2>};
     static constexpr SolveFn w4[3] = {solve_wide_kernel<4, 0>, solve_wide_kernel<4, 1>,
                                       solve_wide_kernel<4, 2>};
+    if (f64 && mode == MODE_TRIDIAG) {
+      if (pad == 1) return solve_wide_kernel<1, 0, double>;
+      if (pad == 4) return solve_wide_kernel<4, 0, double>;
+    }
     if (pad == 1) return w1[mode];
     if (pad == 4) return w4[mode];
     return nullptr;
@@ -920,18 +1400,141 @@ static SolveFn pick(int group, int pad, int mode) {
   return nullptr;
 }
 
-bool solve_variant_exists(int group, int pad) { return pick(group, pad, 0) != nullptr; }
+bool solve_variant_exists(int group, int pad) { return pick(group, pad, 0, false) != nullptr; }
 
-hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStream_t s) {
+// workgroups of a variant that fit on the device at once (cached per variant)
+static int resident_blocks(SolveFn fn, int threads) {
+  static std::mutex mu;
+  static std::map<SolveFn, int> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(fn);
+  if (it != cache.end()) return it->second;
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, threads, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    return 0;
+  const int r = cus * per_cu;
+  cache[fn] = r;
+  return r;
+}
+
+hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStream_t s,
+                        int64_t max_slots, bool f64) {
   if (mode < 0 || mode > 2) return hipErrorInvalidValue;
-  SolveFn fn = pick(group, pad, mode);
+  SolveFn fn = pick(group, pad, mode, f64);
   if (!fn) return hipErrorInvalidValue;
-  if (a.nslots <= 0) return hipSuccess;
-  const int threads = group == WIDE ? WIDE : BLOCK;
-  const int spb = group == WIDE ? 1 : BLOCK / group;
-  const int64_t blocks = (a.nslots + spb - 1) / spb;
+  const int threads = group == K2 ? K2T : group == WIDE ? WIDE : BLOCK;
+  const int spb = group >= WIDE ? 1 : BLOCK / group;
+  int64_t blocks;
+  if (a.bins) {
+    // persistent: as many workgroups as are resident, never more than the
+    // bin can use
+    const int res = resident_blocks(fn, threads);
+    if (res <= 0) return hipErrorInvalidConfiguration;
+    blocks = std::min<int64_t>(res, (max_slots + spb - 1) / spb);
+  } else {
+    blocks = (a.nslots + spb - 1) / spb;
+  }
+  if (blocks <= 0) return hipSuccess;
   if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
   hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(threads), 0, s, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Binning of subsets by size: count per bin (LDS histogram per block, one
+// global atomic per bin and block), exclusive scan, scatter of subset ids into
+// the order array (LDS atomics give each subset its place within the block's
+// share of the bin, one global atomic per bin and block reserves that share).
+// Subsets keep their own results whatever slot they land in, so the order
+// inside a bin does not matter.
+// ---------------------------------------------------------------------------
+constexpr int BIN_THREADS = 256;
+constexpr int BIN_ITEMS = 8;
+
+__device__ __forceinline__ int bin_of(const int64_t *offsets, int kuni, const BinLut &lut,
+                                      int64_t s) {
+  const int64_t k = offsets ? offsets[s + 1] - offsets[s] : (int64_t)kuni;
+  return lut.bin[lut_index(k)];
+}
+
+__global__ void bin_clear_kernel(int32_t *table) {
+  for (int i = threadIdx.x; i < 3 * MAXBIN; i += blockDim.x) table[i] = 0;
+}
+
+__global__ __launch_bounds__(BIN_THREADS) void bin_count_kernel(const int64_t *offsets, int kuni,
+                                                                int64_t nsub, BinLut lut,
+                                                                int nbins, int32_t *table) {
+  __shared__ int h[MAXBIN];
+  for (int i = threadIdx.x; i < nbins; i += BIN_THREADS) h[i] = 0;
+  __syncthreads();
+  for (int64_t s = (int64_t)blockIdx.x * BIN_THREADS + threadIdx.x; s < nsub;
+       s += (int64_t)gridDim.x * BIN_THREADS)
+    atomicAdd(&h[bin_of(offsets, kuni, lut, s)], 1);
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbins; i += BIN_THREADS)
+    if (h[i]) atomicAdd(&table[BIN_COUNT + i], h[i]);
+}
+
+__global__ void bin_scan_kernel(int32_t *table, int nbins) {
+  if (threadIdx.x != 0) return;
+  int acc = 0;
+  for (int b = 0; b < nbins; ++b) {
+    table[BIN_START + b] = acc;
+    table[BIN_FILL + b] = 0;
+    acc += table[BIN_COUNT + b];
+  }
+}
+
+__global__ __launch_bounds__(BIN_THREADS) void bin_scatter_kernel(const int64_t *offsets,
+                                                                  int kuni, int64_t nsub,
+                                                                  BinLut lut, int nbins,
+                                                                  int32_t *table,
+                                                                  int32_t *order) {
+  __shared__ int h[MAXBIN];
+  __shared__ int base[MAXBIN];
+  for (int i = threadIdx.x; i < nbins; i += BIN_THREADS) h[i] = 0;
+  __syncthreads();
+  const int64_t t0 = (int64_t)blockIdx.x * BIN_THREADS * BIN_ITEMS;
+  int b[BIN_ITEMS], loc[BIN_ITEMS];
+#pragma unroll
+  for (int u = 0; u < BIN_ITEMS; ++u) {
+    const int64_t s = t0 + u * BIN_THREADS + threadIdx.x;
+    b[u] = 0;
+    loc[u] = 0;
+    if (s < nsub) {
+      b[u] = bin_of(offsets, kuni, lut, s);
+      loc[u] = atomicAdd(&h[b[u]], 1);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbins; i += BIN_THREADS)
+    if (h[i]) base[i] = table[BIN_START + i] + atomicAdd(&table[BIN_FILL + i], h[i]);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < BIN_ITEMS; ++u) {
+    const int64_t s = t0 + u * BIN_THREADS + threadIdx.x;
+    if (s < nsub) order[base[b[u]] + loc[u]] = (int32_t)s;
+  }
+}
+
+hipError_t launch_binning(const int64_t *offsets, int kuni, int64_t nsub, const BinLut &lut,
+                          int nbins, int32_t *table, int32_t *order, hipStream_t s) {
+  if (nbins < 1 || nbins > MAXBIN || nsub < 0 || nsub > 0x7fffffffLL)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(bin_clear_kernel, dim3(1), dim3(256), 0, s, table);
+  if (nsub == 0) return hipGetLastError();
+  const int64_t cblocks = std::min<int64_t>((nsub + BIN_THREADS - 1) / BIN_THREADS, 1024);
+  hipLaunchKernelGGL(bin_count_kernel, dim3((unsigned)cblocks), dim3(BIN_THREADS), 0, s, offsets,
+                     kuni, nsub, lut, nbins, table);
+  hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(64), 0, s, table, nbins);
+  const int64_t tile = (int64_t)BIN_THREADS * BIN_ITEMS;
+  const int64_t sblocks = (nsub + tile - 1) / tile;
+  hipLaunchKernelGGL(bin_scatter_kernel, dim3((unsigned)sblocks), dim3(BIN_THREADS), 0, s,
+                     offsets, kuni, nsub, lut, nbins, table, order);
   return hipGetLastError();
 }
 
@@ -1011,12 +1614,17 @@ __device__ __forceinline__ int subset_k(const BisectArgs &a, int64_t s) {
   return a.offsets ? (int)(a.offsets[s + 1] - a.offsets[s]) : a.kuni;
 }
 
-// copy the T of subsets s0 .. s0+cnt-1 into the LDS stage (one row per subset)
+__device__ __forceinline__ int64_t subset_at(const BisectArgs &a, int64_t slot) {
+  return a.order ? (int64_t)a.order[slot] : slot;
+}
+
+// copy the T of the subsets in slots s0 .. s0+cnt-1 into the LDS stage (one
+// row per subset)
 __device__ __forceinline__ void stage_tridiag(const BisectArgs &a, int64_t s0, int cnt,
                                               float2 *stage) {
   const int rs = a.tstride + 1;
   for (int j = 0; j < cnt; ++j) {
-    const float2 *src = a.tri + (s0 + j) * (int64_t)a.tstride;
+    const float2 *src = a.tri + subset_at(a, s0 + j) * (int64_t)a.tstride;
     for (int i = threadIdx.x; i < a.tstride; i += BIS_BLOCK) stage[j * rs + i] = src[i];
   }
   __syncthreads();
@@ -1076,7 +1684,7 @@ __global__ __launch_bounds__(BIS_BLOCK) void bisect_extreme_kernel(BisectArgs a)
   stage_tridiag(a, s0, cnt, stage);
   const int j = threadIdx.x;
   if (j >= cnt) return;
-  const int64_t s = s0 + j;
+  const int64_t s = subset_at(a, s0 + j);
   const int k = subset_k(a, s);
   uint32_t st = a.status[s];
   double lmin = NAN, lmax = NAN;

@@ -14,6 +14,7 @@ import torch
 from ._lib import ParaLanczosError, check, load
 
 FORMS = {"tuned": 0, "plain": 1}
+REORTH = {None: 0, "auto": 0, "f32": 0x100, "f64": 0x200}
 PHI_KINDS = {"gauss": 0, "bernoulli": 1, "dct": 2}
 
 SUB_BREAKDOWN = 1
@@ -97,9 +98,12 @@ def gram(phiT: torch.Tensor) -> torch.Tensor:
     return G
 
 
-def solve(G, idx, k=None, offsets=None, form="tuned", lanes=None, pad=None, out=None):
+def solve(G, idx, k=None, offsets=None, form="tuned", lanes=None, pad=None, out=None,
+          reorth=None):
     """lambda_min, lambda_max, delta (FP32) and status (int32 view of the u32
-    status word) per subset. Uniform size k, or CSR offsets (k = largest size)."""
+    status word) per subset. Uniform size k, or CSR offsets (k = largest size).
+    reorth: None/'auto' (FP64 Gram-Schmidt coefficients from k = 128 on in the
+    tuned form), 'f32' or 'f64' (forced for k > 64)."""
     L = load()
     _dev(G, torch.float32, "G")
     _dev(idx, torch.int32, "idx")
@@ -127,7 +131,7 @@ def solve(G, idx, k=None, offsets=None, form="tuned", lanes=None, pad=None, out=
         pad = pad if pad is not None else 4
         check(L.pl_solve_variant(*args, int(lanes), int(pad), _stream()), "pl_solve_variant")
     else:
-        check(L.pl_solve(*args, FORMS[form], _stream()), "pl_solve")
+        check(L.pl_solve(*args, FORMS[form] | REORTH[reorth], _stream()), "pl_solve")
     return lmin, lmax, delta, status
 
 
@@ -157,3 +161,88 @@ def gather(G, idx, k, mode="dump", lanes=0, pad=4):
     check(L.pl_gather(_p(G), G.shape[0], G.stride(0), _p(idx), int(k), nsub, _p(out),
                       0 if mode == "dump" else 1, int(lanes), int(pad), _stream()), "pl_gather")
     return out
+
+
+class Plan:
+    """A solve bound to fixed device tensors (``pl_plan_*``): the first
+    ``execute`` of a tuned plan records a HIP graph, later ones replay it. The
+    tensors' contents may change between executions, their storage may not."""
+
+    def __init__(self, G, idx, k=None, offsets=None, form="tuned", reorth=None):
+        L = load()
+        _dev(G, torch.float32, "G")
+        _dev(idx, torch.int32, "idx")
+        if offsets is not None:
+            _dev(offsets, torch.int64, "offsets")
+            nsub = offsets.numel() - 1
+            if k is None:
+                raise ParaLanczosError("k (the largest subset size) is required")
+        else:
+            if k is None:
+                raise ParaLanczosError("k is required without offsets")
+            nsub = idx.numel() // k
+        self.nsub = nsub
+        self.lmin = torch.empty(nsub, dtype=torch.float32, device=G.device)
+        self.lmax = torch.empty_like(self.lmin)
+        self.delta = torch.empty_like(self.lmin)
+        self.status = torch.empty(nsub, dtype=torch.int32, device=G.device)
+        # keep the bound tensors alive as long as the plan
+        self._keep = (G, idx, offsets)
+        h = ctypes.c_void_p()
+        check(L.pl_plan_create(ctypes.byref(h), _p(G), G.shape[0], G.stride(0), _p(idx),
+                               _p(offsets), int(k), nsub, _p(self.lmin), _p(self.lmax),
+                               _p(self.delta), _p(self.status), FORMS[form] | REORTH[reorth]),
+              "pl_plan_create")
+        self._h = h
+
+    def execute(self):
+        check(load().pl_plan_execute(self._h, _stream()), "pl_plan_execute")
+        return self.lmin, self.lmax, self.delta, self.status
+
+    @property
+    def is_graph(self) -> bool:
+        return bool(load().pl_plan_is_graph(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            check(load().pl_plan_destroy(self._h), "pl_plan_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(G, klist, sample=65536, seed=0x7A, stream=True):
+    """Time every (lanes, pad) variant per size in klist on seeded subsets of
+    G and store the winners in the library's tuning table. Returns
+    {k: (lanes, pad, ms)}."""
+    L = load()
+    _dev(G, torch.float32, "G")
+    ks = [int(x) for x in klist]
+    kl = (ctypes.c_int32 * len(ks))(*ks)
+    best = (ctypes.c_double * len(ks))()
+    check(L.pl_autotune(_p(G), G.shape[0], G.stride(0), ctypes.cast(kl, ctypes.c_void_p),
+                        len(ks), int(sample), int(seed), ctypes.cast(best, ctypes.c_void_p),
+                        _stream()), "pl_autotune")
+    out = {}
+    for i, k in enumerate(ks):
+        lanes, pad = tuning_get(k)
+        out[k] = (lanes, pad, float(best[i]))
+    return out
+
+
+def tuning_get(k):
+    lanes, pad = ctypes.c_int(), ctypes.c_int()
+    check(load().pl_tuning_get(int(k), ctypes.byref(lanes), ctypes.byref(pad)), "pl_tuning_get")
+    return lanes.value, pad.value
+
+
+def tuning_set(k, lanes, pad):
+    check(load().pl_tuning_set(int(k), int(lanes), int(pad)), "pl_tuning_set")
+
+
+def tuning_reset():
+    check(load().pl_tuning_reset(), "pl_tuning_reset")

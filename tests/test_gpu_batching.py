@@ -1,0 +1,144 @@
+"""Mixed-size batching on the GPU: device-side binning, persistent per-bin
+launches, graph-captured plans and the autotuner.
+
+Every form computes each subset with the same arithmetic (zero padding makes
+the result independent of the lane count and LDS padding), so the tuned
+binned form, the plain one-launch-per-k form, a single explicit variant and
+the graph replays must agree bit for bit; the oracle pins the values
+themselves (relative error <= 1e-5).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from paralanczos_amd import solver
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def rel(x, ref):
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return np.abs(x - ref) / np.maximum(np.abs(ref), 1e-30)
+
+
+@pytest.fixture(scope="module")
+def mixed(gpu):
+    n = 2048
+    G = solver.gram(solver.gen_phi("gauss", 0xC3, 512, n))
+    ks = solver.gen_ks(0xB1, 3000, list(range(8, 129, 8)) + [1, 3, 5, 13, 31, 33, 65, 127])
+    off = solver.offsets_from_ks(ks)
+    idx = solver.gen_subsets_csr(0xB2, off, n)
+    torch.cuda.synchronize()
+    return G, ks, off, idx
+
+
+def _np(ts):
+    return [t.cpu().numpy() for t in ts]
+
+
+def test_tuned_plain_variant_bitexact(mixed):
+    """With FP32 Gram-Schmidt coefficients at every k (the tuned form switches
+    to FP64 ones at k = 128 by default) the three forms agree bit for bit."""
+    G, ks, off, idx = mixed
+    tuned = _np(solver.solve(G, idx, k=128, offsets=off, form="tuned", reorth="f32"))
+    plain = _np(solver.solve(G, idx, k=128, offsets=off, form="plain"))
+    single = _np(solver.solve(G, idx, k=128, offsets=off, lanes=128, pad=4))
+    for a, b, c in zip(tuned, plain, single):
+        assert np.array_equal(a, b, equal_nan=True)
+        assert np.array_equal(a, c, equal_nan=True)
+    assert np.all(tuned[3] == 0)
+    # the default tuned form differs only in the k = 128 subsets, within 1e-5
+    dflt = _np(solver.solve(G, idx, k=128, offsets=off))
+    kk = ks.cpu().numpy()
+    assert np.array_equal(dflt[0][kk < 128], tuned[0][kk < 128])
+    assert rel(dflt[0], tuned[0]).max() <= RTOL and rel(dflt[1], tuned[1]).max() <= RTOL
+    # the values: the oracle on a slice of the batch
+    o = off.cpu().numpy()
+    sl = slice(0, 400)
+    olo, ohi, _, ost = oracle.solve_batch(G.cpu().numpy(), idx.cpu().numpy()[:o[400]],
+                                          offsets=o[:401], nthreads=8)
+    assert np.all(ost == 0)
+    assert rel(tuned[0][sl], olo).max() <= RTOL and rel(tuned[1][sl], ohi).max() <= RTOL
+
+
+def test_sizes_beyond_kmax_flagged(mixed):
+    """k larger than the declared maximum is BADK (its T would not fit the
+    stride), sizes up to it are solved, in every form."""
+    G, ks, off, idx = mixed
+    kmax = 40
+    kk = ks.cpu().numpy()
+    for form in ("tuned", "plain"):
+        lo, hi, de, st = _np(solver.solve(G, idx, k=kmax, offsets=off, form=form))
+        big = kk > kmax
+        assert np.all(st[big] == 16) and np.all(np.isnan(lo[big]))
+        assert np.all(st[~big] == 0) and np.all(np.isfinite(lo[~big]))
+    ref = _np(solver.solve(G, idx, k=128, offsets=off))
+    assert np.array_equal(lo[~big], ref[0][~big])
+
+
+def test_plan_graph_replay(mixed):
+    """A tuned plan records a graph on its first execution and replays it;
+    every replay, with new index sets written into the bound tensor and eager
+    device work in between, equals an eager solve of the same inputs."""
+    G, ks, off, idx = mixed
+    n = G.shape[0]
+    idx2 = idx.clone()
+    plan = solver.Plan(G, idx2, k=128, offsets=off, form="tuned")
+    try:
+        for rep in range(4):
+            idx2.copy_(solver.gen_subsets_csr(0xB2 + rep, off, n))
+            junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+            got = _np(plan.execute())
+            torch.cuda.synchronize()
+            assert plan.is_graph
+            ref = _np(solver.solve(G, idx2, k=128, offsets=off))
+            for a, b in zip(got, ref):
+                assert np.array_equal(a, b, equal_nan=True), rep
+            assert np.isfinite(junk.item())
+    finally:
+        plan.close()
+
+
+def test_plan_uniform_and_plain(mixed):
+    G, ks, off, idx = mixed
+    n = G.shape[0]
+    u = solver.gen_subsets(0xB7, 5000, 24, n)
+    plan = solver.Plan(G, u, k=24, form="tuned")
+    ref = _np(solver.solve(G, u, k=24))
+    for _ in range(3):
+        got = _np(plan.execute())
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b, equal_nan=True)
+    assert plan.is_graph
+    plan.close()
+    # the plain form of a mixed batch waits for bin counts: it runs eagerly
+    pp = solver.Plan(G, idx, k=128, offsets=off, form="plain")
+    got = _np(pp.execute())
+    assert not pp.is_graph
+    ref = _np(solver.solve(G, idx, k=128, offsets=off, form="plain"))
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b, equal_nan=True)
+    pp.close()
+
+
+def test_autotune(mixed):
+    G, ks, off, idx = mixed
+    before = _np(solver.solve(G, idx, k=128, offsets=off))
+    try:
+        res = solver.autotune(G, [8, 24, 40, 72], sample=4096)
+        for k, (lanes, pad, ms) in res.items():
+            assert lanes >= k and pad in (0, 1, 4) and ms > 0
+        # unlisted sizes take the winner of the next listed size above them
+        for k in range(1, 73):
+            nxt = min(x for x in (8, 24, 40, 72) if x >= k)
+            assert solver.tuning_get(k) == solver.tuning_get(nxt)[:2]
+        after = _np(solver.solve(G, idx, k=128, offsets=off))
+        for a, b in zip(before, after):
+            assert np.array_equal(a, b, equal_nan=True)
+    finally:
+        solver.tuning_reset()
+    assert solver.tuning_get(24) == (32, 4)

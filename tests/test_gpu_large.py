@@ -1,0 +1,136 @@
+"""Subsets of 128 < k <= 256 (the 512-lane kernel with its basis split between
+LDS and registers) and the FP64 Gram-Schmidt coefficient forms, against the
+oracle and FP64 LAPACK (relative eigenvalue error <= 1e-5)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from paralanczos_amd import solver
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def rel(x, ref):
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return np.abs(x - ref) / np.maximum(np.abs(ref), 1e-30)
+
+
+def _np(ts):
+    return [t.cpu().numpy() for t in ts]
+
+
+def test_gather_dump_k256_bit_exact(gpu):
+    n = 1024
+    G = solver.gram(solver.gen_phi("gauss", 3, 256, n))
+    Gh = G.cpu().numpy()
+    for k in (129, 200, 256):
+        idx = solver.gen_subsets(40 + k, 9, k, n)
+        out = solver.gather(G, idx, k, "dump", 256, 4).cpu().numpy().reshape(-1, k, k)
+        ih = idx.cpu().numpy()
+        for s in range(9):
+            A, st = oracle.gather(Gh, ih[s * k:(s + 1) * k])
+            assert st == 0 and np.array_equal(out[s], A.astype(np.float32)), (k, s)
+        cs = solver.gather(G, idx, k, "checksum", 256, 4).cpu().numpy()
+        np.testing.assert_allclose(cs, out.reshape(9, -1).astype(np.float64).sum(1), rtol=1e-4)
+
+
+@pytest.mark.parametrize("k", [100, 129, 160, 200, 255, 256])
+def test_k256_vs_oracle(gpu, k):
+    n = 1024
+    G = solver.gram(solver.gen_phi("gauss", 900 + k, 4 * k, n))
+    idx = solver.gen_subsets(77 + k, 24, k, n)
+    Gh = G.cpu().numpy()
+    ih = idx.cpu().numpy()
+    olo, ohi, _, ost = oracle.solve_batch(Gh, ih, k=k, nthreads=8)
+    assert np.all(ost == 0)
+    runs = [solver.solve(G, idx, k=k, lanes=256, pad=4)]
+    if k > 128:
+        runs += [solver.solve(G, idx, k=k, reorth=r) for r in ("f32", "f64", None)]
+    for r in runs:
+        lo, hi, de, st = _np(r)
+        assert np.all(st == 0)
+        assert rel(lo, olo).max() <= RTOL and rel(hi, ohi).max() <= RTOL
+        np.testing.assert_allclose(de, np.maximum(hi - 1.0, 1.0 - lo), rtol=1e-6, atol=1e-7)
+
+
+def test_k256_rank_deficient_alleig(gpu):
+    """m = 160 < k: G_S has rank 160 and the Krylov space of the start vector
+    is exhausted early. Whether beta then falls under the restart threshold
+    (flagged BREAKDOWN) or stays at FP32 rounding level (the reorthogonalised
+    remainder continues the recurrence), every eigenvalue agrees with LAPACK,
+    the zero ones to 1e-5 of lambda_max."""
+    n = 600
+    G = solver.gram(solver.gen_phi("gauss", 5, 160, n))
+    Gh = G.cpu().numpy()
+    for k in (200, 256):
+        idx = solver.gen_subsets(6 + k, 8, k, n)
+        eig, det, err = _np(solver.solve_alleig(G, idx, k))
+        ih = idx.cpu().numpy()
+        assert np.all((err & ~1) == 0)
+        for s in range(8):
+            A, _ = oracle.gather(Gh, ih[s * k:(s + 1) * k])
+            w = np.linalg.eigvalsh(A)
+            e = eig[s * k:(s + 1) * k]
+            assert np.all(np.diff(e) >= 0)
+            assert np.max(np.abs(e - w)) <= 1e-5 * w[-1], (k, s)
+
+
+def test_alleig_k256_full_rank(gpu):
+    n = 2048
+    G = solver.gram(solver.gen_phi("gauss", 8, 1024, n))
+    Gh = G.cpu().numpy()
+    k = 224
+    idx = solver.gen_subsets(9, 6, k, n)
+    eig, det, err = _np(solver.solve_alleig(G, idx, k))
+    ih = idx.cpu().numpy()
+    assert np.all(err == 0)
+    for s in range(6):
+        A, _ = oracle.gather(Gh, ih[s * k:(s + 1) * k])
+        w = np.linalg.eigvalsh(A)
+        assert np.max(np.abs(eig[s * k:(s + 1) * k] - w)) <= 1e-5 * w[-1]
+        assert rel(det[s], np.prod(w)) <= 1e-3
+
+
+def test_mixed_up_to_256(gpu):
+    """k from {16, 32, 64, 128, 256} in one CSR batch: the tuned form (FP64
+    coefficients from 128) within 1e-5 of the oracle; with FP32 coefficients
+    forced it is bit-identical to the plain one-launch-per-k form."""
+    n = 4096
+    G = solver.gram(solver.gen_phi("gauss", 0xC5, 1024, n))
+    ks = solver.gen_ks(0xC5 + 1, 600, [16, 32, 64, 128, 256])
+    off = solver.offsets_from_ks(ks)
+    idx = solver.gen_subsets_csr(0xC5 + 2, off, n)
+    tuned = _np(solver.solve(G, idx, k=256, offsets=off))
+    plain = _np(solver.solve(G, idx, k=256, offsets=off, form="plain"))
+    t32 = _np(solver.solve(G, idx, k=256, offsets=off, reorth="f32"))
+    for a, b in zip(t32, plain):
+        assert np.array_equal(a, b)
+    o = off.cpu().numpy()
+    olo, ohi, _, ost = oracle.solve_batch(G.cpu().numpy(), idx.cpu().numpy(), offsets=o,
+                                          nthreads=8)
+    assert np.all(ost == 0)
+    for r in (tuned, plain):
+        assert np.all(r[3] == 0)
+        assert rel(r[0], olo).max() <= RTOL and rel(r[1], ohi).max() <= RTOL
+
+
+def test_c4_bernoulli_sample(gpu):
+    """The precision workload's shape: Bernoulli +-1/sqrt(2048) Phi of
+    2048 x 16384, G = Phi^T Phi (1 GiB), k = 128 and 256; both coefficient
+    precisions within 1e-5 of dsyevd on the gathered submatrices."""
+    m, n = 2048, 16384
+    G = solver.gram(solver.gen_phi("bernoulli", 0xC4, m, n))
+    for k in (128, 256):
+        idx = solver.gen_subsets(0xC4 + 2, 48, k, n)
+        A = solver.gather(G, idx, k, "dump", 256, 4).cpu().numpy().reshape(-1, k, k)
+        w = np.linalg.eigvalsh(A.astype(np.float64))
+        for r in ("f32", "f64"):
+            lo, hi, de, st = _np(solver.solve(G, idx, k=k, reorth=r))
+            assert np.all(st == 0), (k, r)
+            assert rel(lo, w[:, 0]).max() <= RTOL and rel(hi, w[:, -1]).max() <= RTOL, (k, r)
+    del G
+    torch.cuda.empty_cache()
