@@ -140,6 +140,51 @@ pl_status pl_tuning_get(int k, int *lanes, int *pad);
 pl_status pl_tuning_set(int k, int lanes, int pad);
 pl_status pl_tuning_reset(void);
 
+/*
+ * Serving engine (no reference counterpart): continuous batching of requests
+ * against a device-resident G. A request is nsub subsets whose index sets
+ * (idx, nidx entries), CSR offsets (request-local, offsets[0] = 0 and
+ * offsets[nsub] = nidx; or NULL with uniform size k) and outputs (lmin, lmax, delta, status; nsub entries
+ * each) are caller-owned DEVICE buffers that must stay valid until the
+ * request is waited for; kmax is the largest subset size in it. Requests are
+ * packed in arrival order into steps bounded by the per-step arena
+ * (max_subsets, max_indices, max_requests) and by work_budget (sum over the
+ * requests of nsub * LDS bytes per subset of the kernel for kmax * kmax; 0 =
+ * no bound); a request alone in a step is always admitted. Up to `depth`
+ * steps run at once, each on its own stream. Results equal those of pl_solve
+ * with the same form, whatever the mix. Submission and waiting are
+ * thread-safe; pl_server_destroy finishes every queued request first.
+ */
+typedef struct pl_server pl_server;
+typedef struct {
+  int64_t max_subsets;  /* subsets per step (device arena)          */
+  int64_t max_indices;  /* index entries per step                   */
+  int32_t max_requests; /* requests per step                        */
+  int32_t max_k;        /* largest subset size accepted (<= pl_max_k) */
+  int32_t depth;        /* steps in flight, 1..8                    */
+  int32_t form;         /* PL_FORM_TUNED or PL_FORM_PLAIN (| precision flag) */
+  double work_budget;   /* per-step LDS-occupancy budget, 0 = none  */
+} pl_server_config;
+typedef struct {
+  int64_t requests, subsets, steps; /* completed since the last reset */
+  double span_s;                    /* first submission to last completion */
+  double p50_s, p99_s, max_s;       /* request latency, submission to completion */
+} pl_server_stats;
+pl_status pl_server_create(const float *G, int64_t n, int64_t ld, const pl_server_config *cfg,
+                           pl_server **server);
+/* `stream` is the stream whose work writes the request's inputs (NULL = the
+   default stream): the request's step starts after that work, as if ordered
+   on it. Outputs are complete when pl_server_wait returns. */
+pl_status pl_server_submit(pl_server *server, const int32_t *idx, const int64_t *offsets, int k,
+                           int64_t nsub, int64_t nidx, int kmax, float *lmin, float *lmax,
+                           float *delta, uint32_t *status, void *stream, int64_t *ticket);
+/* block until the request is done; latency_s = submission to completion,
+   queued_s = submission to the start of its step (either may be NULL) */
+pl_status pl_server_wait(pl_server *server, int64_t ticket, double *latency_s, double *queued_s);
+pl_status pl_server_poll(pl_server *server, int64_t ticket, int *done);
+pl_status pl_server_query(pl_server *server, pl_server_stats *stats, int reset);
+pl_status pl_server_destroy(pl_server *server);
+
 /* all k eigenvalues ascending eig[s*k + j] (FP64), det[s] = prod_j eig, and
    err[s] = per-subset status, for nsub subsets of size k (uniform) */
 pl_status pl_solve_alleig(const float *G, int64_t n, int64_t ld, const int32_t *idx, int k,

@@ -35,6 +35,12 @@ SIGNATURES = {
     "pl_tuning_get": (ST, [INT, P, P]),
     "pl_tuning_set": (ST, [INT, INT, INT]),
     "pl_tuning_reset": (ST, []),
+    "pl_server_create": (ST, [P, I64, I64, P, P]),
+    "pl_server_submit": (ST, [P, P, P, INT, I64, I64, INT, P, P, P, P, P, P]),
+    "pl_server_wait": (ST, [P, I64, P, P]),
+    "pl_server_poll": (ST, [P, I64, P]),
+    "pl_server_query": (ST, [P, P, INT]),
+    "pl_server_destroy": (ST, [P]),
     "pl_gather": (ST, [P, I64, I64, P, INT, I64, P, INT, INT, INT, P]),
     "pl_gen_subsets": (ST, [U64, I64, I64, INT, INT, P, P]),
     "pl_gen_subsets_csr": (ST, [U64, I64, I64, P, INT, P, P]),
@@ -48,6 +54,18 @@ SIGNATURES = {
     "calculate_subeigs3": (None, [P, INT, INT, P, P, P, INT]),
     "pl_multi_subsvd": (INT, [P, INT, P, INT, INT, P, P, P]),
 }
+
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_subsets", I64), ("max_indices", I64), ("max_requests", I32),
+                ("max_k", I32), ("depth", I32), ("form", I32), ("work_budget", DBL)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests", I64), ("subsets", I64), ("steps", I64), ("span_s", DBL),
+                ("p50_s", DBL), ("p99_s", DBL), ("max_s", DBL)]
+
 
 _lib = None
 

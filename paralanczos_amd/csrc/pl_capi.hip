@@ -49,10 +49,7 @@ int group_for(int k) {
 // each k by the tuned form. Defaults: the smallest lane group >= k, pad 4;
 // pl_autotune overwrites entries with measured winners.
 // ---------------------------------------------------------------------------
-struct Variant {
-  int lanes, pad;
-  bool f64 = false;  // FP64 Gram-Schmidt coefficients
-};
+using pl::Variant;
 std::mutex g_tune_mu;
 Variant g_tune[pl::KMAX_ALL + 1];
 bool g_tune_ready = false;
@@ -170,20 +167,7 @@ struct Workspace {
   }
 };
 
-// everything one solve call needs, bound to device buffers
-struct SolveJob {
-  const float *G;
-  int64_t n, ld;
-  const int32_t *idx;
-  const int64_t *offsets;
-  int k;  // uniform size, or the largest size of a mixed batch
-  int64_t nsub;
-  float *lmin, *lmax, *delta;
-  uint32_t *status;
-  int form;           // PL_FORM_TUNED / PL_FORM_PLAIN (| precision flags), or -1 for an
-                      // explicit variant
-  Variant explicit_v;  // used when form == -1
-};
+using pl::SolveJob;
 
 pl::SolveArgs solve_args(const SolveJob &j, const pl::WorkspaceParts &w) {
   pl::SolveArgs a{};
@@ -312,6 +296,19 @@ pl_status solve_once(const SolveJob &j, void *ws, size_t ws_bytes, void *stream)
 }
 
 }  // namespace
+
+namespace pl {
+pl_status solve_job(const SolveJob &j, void *ws, hipStream_t s) { return run_job(j, ws, s); }
+pl_status validate_job(const SolveJob &j) { return check_job(j); }
+pl_status set_error(pl_status st, const char *msg) { return fail(st, "%s", msg); }
+pl_status set_hip_error(hipError_t e, const char *what) { return hip_fail(e, what); }
+int lds_bytes_of(int form, int k) {
+  const Variant v = job_variant(form, k);
+  if (v.lanes == 256) return 154 * 1024;
+  if (v.lanes == 128) return 128 * (128 + v.pad) * 4;
+  return v.lanes * (v.lanes + v.pad) * 4;
+}
+}  // namespace pl
 
 extern "C" {
 

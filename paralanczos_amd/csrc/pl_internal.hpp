@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/paralanczos.h"
+
 namespace pl {
 
 // what a solve launch writes per subset
@@ -110,5 +112,36 @@ hipError_t launch_gen_ks(uint64_t seed, int64_t s0, int64_t nsub, const int32_t 
 hipError_t launch_gen_phi_gauss(uint64_t seed, int m, int n, float *phi, hipStream_t s);
 hipError_t launch_gen_phi_bernoulli(uint64_t seed, int m, int n, float *phi, hipStream_t s);
 hipError_t launch_gen_phi_dct(const int32_t *rows_dev, int m, int n, float *phi, hipStream_t s);
+
+// ---- host-side solve jobs (pl_capi.hip), shared with the serving engine ----
+struct Variant {
+  int lanes, pad;
+  bool f64 = false;  // FP64 Gram-Schmidt coefficients
+};
+
+// everything one solve call needs, bound to device buffers
+struct SolveJob {
+  const float *G;
+  int64_t n, ld;
+  const int32_t *idx;
+  const int64_t *offsets;
+  int k;  // uniform size, or the largest size of a mixed batch
+  int64_t nsub;
+  float *lmin, *lmax, *delta;
+  uint32_t *status;
+  int form;           // PL_FORM_TUNED / PL_FORM_PLAIN (| precision flags), or -1 for an
+                      // explicit variant
+  Variant explicit_v;  // used when form == -1
+};
+
+// the stream-ordered launches of one solve on a workspace of
+// workspace_bytes(nsub, k) bytes (no host synchronisation except for the
+// plain form of a mixed batch)
+pl_status solve_job(const SolveJob &j, void *ws, hipStream_t s);
+pl_status validate_job(const SolveJob &j);
+pl_status set_error(pl_status st, const char *msg);
+pl_status set_hip_error(hipError_t e, const char *what);
+// LDS bytes the variant for size k occupies per subset
+int lds_bytes_of(int form, int k);
 
 }  // namespace pl

@@ -1365,7 +1365,7 @@ __global__ __launch_bounds__(K2T) void solve_k256_kernel(SolveArgs a) {
 using SolveFn = void (*)(SolveArgs);
 
 template <int GROUP, int PAD>
-struct Variant {
+struct VariantFns {
   static constexpr SolveFn fns[3] = {solve_kernel<GROUP, PAD, 0>, solve_kernel<GROUP, PAD, 1>,
                                      solve_kernel<GROUP, PAD, 2>};
 };
@@ -1391,7 +1391,7 @@ static SolveFn pick(int group, int pad, int mode, bool f64) {
     return nullptr;
   }
 #define PL_CASE(G, P) \
-  if (group == G && pad == P) return Variant<G, P>::fns[mode];
+  if (group == G && pad == P) return VariantFns<G, P>::fns[mode];
   PL_CASE(8, 0) PL_CASE(8, 1) PL_CASE(8, 4)
   PL_CASE(16, 0) PL_CASE(16, 1) PL_CASE(16, 4)
   PL_CASE(32, 0) PL_CASE(32, 1) PL_CASE(32, 4)

@@ -246,3 +246,77 @@ def tuning_set(k, lanes, pad):
 
 def tuning_reset():
     check(load().pl_tuning_reset(), "pl_tuning_reset")
+
+
+class Server:
+    """Continuous-batching serving engine (``pl_server_*``) over a
+    device-resident G. ``submit`` takes device tensors (index sets, and CSR
+    offsets or a uniform k) and returns a ticket; ``wait`` returns the outputs
+    and the request's latency in seconds."""
+
+    def __init__(self, G, max_subsets=65536, max_indices=1 << 23, max_requests=1024,
+                 max_k=256, depth=3, form="tuned", reorth=None, work_budget=0.0):
+        from ._lib import ServerConfig
+        L = load()
+        _dev(G, torch.float32, "G")
+        cfg = ServerConfig(int(max_subsets), int(max_indices), int(max_requests), int(max_k),
+                           int(depth), FORMS[form] | REORTH[reorth], float(work_budget))
+        h = ctypes.c_void_p()
+        check(L.pl_server_create(_p(G), G.shape[0], G.stride(0), ctypes.byref(cfg),
+                                 ctypes.byref(h)), "pl_server_create")
+        self._h = h
+        self._G = G
+        self._pending = {}
+
+    def submit(self, idx, k=None, offsets=None, kmax=None, out=None):
+        _dev(idx, torch.int32, "idx")
+        if offsets is not None:
+            _dev(offsets, torch.int64, "offsets")
+            nsub = offsets.numel() - 1
+            if kmax is None:
+                raise ParaLanczosError("kmax is required with offsets")
+            kk = 0
+        else:
+            if k is None:
+                raise ParaLanczosError("k is required without offsets")
+            nsub = idx.numel() // k
+            kk, kmax = int(k), int(k)
+        dev = idx.device
+        if out is None:
+            out = (torch.empty(nsub, dtype=torch.float32, device=dev),
+                   torch.empty(nsub, dtype=torch.float32, device=dev),
+                   torch.empty(nsub, dtype=torch.float32, device=dev),
+                   torch.empty(nsub, dtype=torch.int32, device=dev))
+        t = ctypes.c_int64()
+        check(load().pl_server_submit(self._h, _p(idx), _p(offsets), kk, nsub, idx.numel(),
+                                      int(kmax), _p(out[0]), _p(out[1]), _p(out[2]),
+                                      _p(out[3]), _stream(), ctypes.byref(t)),
+              "pl_server_submit")
+        self._pending[t.value] = (idx, offsets, out)
+        return t.value
+
+    def wait(self, ticket):
+        lat, q = ctypes.c_double(), ctypes.c_double()
+        check(load().pl_server_wait(self._h, int(ticket), ctypes.byref(lat), ctypes.byref(q)),
+              "pl_server_wait")
+        _, _, out = self._pending.pop(ticket)
+        return out, lat.value, q.value
+
+    def stats(self, reset=False):
+        from ._lib import ServerStats
+        st = ServerStats()
+        check(load().pl_server_query(self._h, ctypes.byref(st), int(bool(reset))),
+              "pl_server_query")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            check(load().pl_server_destroy(self._h), "pl_server_destroy")
+            self._h = None
+            self._pending.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
