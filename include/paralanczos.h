@@ -97,6 +97,16 @@ pl_status pl_solve_ws(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                       float *delta, uint32_t *status, int form, void *workspace,
                       size_t workspace_bytes, void *stream);
 
+/* One stage of pl_solve_ws on a caller workspace: stage 1 gathers each G_S
+   and runs the Lanczos recurrence (the tridiagonal T of every subset and, for
+   a mixed batch, its bins stay in the workspace); stage 2 bisects T for
+   lambda_min, lambda_max and delta. Stage 2 must follow stage 1 with the same
+   arguments and workspace (on one stream, or ordered by the caller). */
+pl_status pl_solve_stage(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                         const int64_t *offsets, int k, int64_t nsub, float *lmin, float *lmax,
+                         float *delta, uint32_t *status, int form, int stage, void *workspace,
+                         size_t workspace_bytes, void *stream);
+
 /* same, with an explicit variant: lanes per subset (8/16/32/64, >= k) and LDS
    row padding (0, 1 or 4 floats) - used by the autotuner */
 pl_status pl_solve_variant(const float *G, int64_t n, int64_t ld, const int32_t *idx,

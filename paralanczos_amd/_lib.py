@@ -25,6 +25,8 @@ SIGNATURES = {
     "pl_solve": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, P]),
     "pl_solve_workspace_size": (ctypes.c_size_t, [I64, INT]),
     "pl_solve_ws": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, P, ctypes.c_size_t, P]),
+    "pl_solve_stage": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, INT, P,
+                            ctypes.c_size_t, P]),
     "pl_solve_variant": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, INT, P]),
     "pl_solve_alleig": (ST, [P, I64, I64, P, INT, I64, P, P, P, P]),
     "pl_plan_create": (ST, [P, P, I64, I64, P, P, INT, I64, P, P, P, P, INT]),

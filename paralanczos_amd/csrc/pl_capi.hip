@@ -218,22 +218,22 @@ bool job_is_async(const SolveJob &j) {
 //     bin draining its run of the order array, bisection in bin order;
 //   mixed, plain: binning by exact k, bin counts to the host, one launch per
 //     k sized to its count, bisection in bin order.
-pl_status run_job(const SolveJob &j, void *ws, hipStream_t s) {
+constexpr int kStageTridiag = 1, kStageBisect = 2;
+
+pl_status run_job(const SolveJob &j, void *ws, hipStream_t s, int stages = 3) {
   const pl::WorkspaceParts w = pl::split_workspace(ws, j.nsub, j.k);
   pl::SolveArgs a = solve_args(j, w);
   hipError_t e;
-  if (!j.offsets) {
-    const Variant v = j.form == -1 ? j.explicit_v : job_variant(j.form, j.k);
-    e = pl::launch_solve(v.lanes, v.pad, pl::MODE_TRIDIAG, a, s, 0, v.f64);
-    if (e != hipSuccess) return hip_fail(e, "solve launch");
-    return run_bisect(j, w, nullptr, s);
+  if (!j.offsets || j.form == -1) {
+    // uniform size, or an explicit variant over a mixed batch: one launch
+    if (stages & kStageTridiag) {
+      const Variant v = j.form == -1 ? j.explicit_v : job_variant(j.form, j.k);
+      e = pl::launch_solve(v.lanes, v.pad, pl::MODE_TRIDIAG, a, s, 0, v.f64);
+      if (e != hipSuccess) return hip_fail(e, "solve launch");
+    }
+    return (stages & kStageBisect) ? run_bisect(j, w, nullptr, s) : PL_OK;
   }
-  if (j.form == -1) {  // explicit variant over a mixed batch: one launch, no binning
-    e = pl::launch_solve(j.explicit_v.lanes, j.explicit_v.pad, pl::MODE_TRIDIAG, a, s, 0,
-                         j.explicit_v.f64);
-    if (e != hipSuccess) return hip_fail(e, "solve launch");
-    return run_bisect(j, w, nullptr, s);
-  }
+  if (!(stages & kStageTridiag)) return (stages & kStageBisect) ? run_bisect(j, w, w.order, s) : PL_OK;
   BinPlan bp;
   make_bins(j.form, j.k, bp);
   e = pl::launch_binning(j.offsets, 0, j.nsub, bp.lut, bp.nbins, w.table, w.order, s);
@@ -263,7 +263,7 @@ pl_status run_job(const SolveJob &j, void *ws, hipStream_t s) {
       if (e != hipSuccess) return hip_fail(e, "per-k solve launch");
     }
   }
-  return run_bisect(j, w, w.order, s);
+  return (stages & kStageBisect) ? run_bisect(j, w, w.order, s) : PL_OK;
 }
 
 pl_status check_job(const SolveJob &j) {
@@ -353,6 +353,23 @@ pl_status pl_solve_ws(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                       size_t workspace_bytes, void *stream) {
   SolveJob j{G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, form, {0, 0, false}};
   return solve_once(j, workspace, workspace_bytes, stream);
+}
+
+pl_status pl_solve_stage(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                         const int64_t *offsets, int k, int64_t nsub, float *lmin, float *lmax,
+                         float *delta, uint32_t *status, int form, int stage, void *workspace,
+                         size_t workspace_bytes, void *stream) {
+  SolveJob j{G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, form, {0, 0, false}};
+  pl_status st = check_job(j);
+  if (st != PL_OK) return st;
+  if (stage != kStageTridiag && stage != kStageBisect)
+    return fail(PL_ERR_ARG, "stage must be 1 (tridiagonalise) or 2 (bisect)");
+  if (!workspace) return fail(PL_ERR_ARG, "pl_solve_stage needs a caller workspace");
+  if (workspace_bytes < pl::workspace_bytes(nsub, k))
+    return fail(PL_ERR_ARG, "workspace of %zu bytes < %zu needed", workspace_bytes,
+                pl::workspace_bytes(nsub, k));
+  if (nsub == 0) return PL_OK;
+  return run_job(j, workspace, (hipStream_t)stream, stage);
 }
 
 pl_status pl_solve_variant(const float *G, int64_t n, int64_t ld, const int32_t *idx,

@@ -135,6 +135,34 @@ def solve(G, idx, k=None, offsets=None, form="tuned", lanes=None, pad=None, out=
     return lmin, lmax, delta, status
 
 
+def workspace_size(nsub, k):
+    return int(load().pl_solve_workspace_size(int(nsub), int(k)))
+
+
+def solve_stage(G, idx, stage, ws, k=None, offsets=None, form="tuned", reorth=None, out=None):
+    """One stage of a solve on the caller's workspace tensor (uint8, at least
+    workspace_size(nsub, k) bytes): 1 = gather + Lanczos (T kept in ws),
+    2 = bisection of T. Returns the output tensors (written by stage 2)."""
+    L = load()
+    _dev(G, torch.float32, "G")
+    _dev(idx, torch.int32, "idx")
+    if offsets is not None:
+        _dev(offsets, torch.int64, "offsets")
+        nsub = offsets.numel() - 1
+    else:
+        nsub = idx.numel() // k
+    if out is None:
+        out = (torch.empty(nsub, dtype=torch.float32, device=G.device),
+               torch.empty(nsub, dtype=torch.float32, device=G.device),
+               torch.empty(nsub, dtype=torch.float32, device=G.device),
+               torch.empty(nsub, dtype=torch.int32, device=G.device))
+    check(L.pl_solve_stage(_p(G), G.shape[0], G.stride(0), _p(idx), _p(offsets), int(k), nsub,
+                           _p(out[0]), _p(out[1]), _p(out[2]), _p(out[3]),
+                           FORMS[form] | REORTH[reorth], int(stage), _p(ws), ws.numel(),
+                           _stream()), "pl_solve_stage")
+    return out
+
+
 def solve_alleig(G, idx, k):
     """All k eigenvalues ascending (FP64), det and status per subset."""
     L = load()
