@@ -131,3 +131,23 @@ def test_print_tridiagonal_layout(capfd):
     assert out == (" [  1 ,  0.5 ,  0 ; \n"
                    " 0.5 ,  2 ,  0.25 ; \n"
                    " 0 ,  0.25 ,  3 ] \n")
+
+
+def test_matlab_wrappers_match_python_restatement():
+    """The .m front-ends carry the same argument messages as the Python
+    restatement of the reference's Matlab checks (no Matlab here to run them)."""
+    import inspect
+    import re
+
+    from paralanczos_amd import legacy
+
+    mdir = os.path.join(ROOT, "paralanczos_amd", "matlab")
+    for name in ("gpu_subsvd", "gpu_multi_subsvd", "gpu_multi_subsvd2", "gpu_svd"):
+        text = open(os.path.join(mdir, name + ".m")).read()
+        assert f"function [eigs, det1, err1] = {name}(" in text
+        m_msgs = [s.replace("''", "'") for s in re.findall(r"disp\('((?:[^']|'')*)'\)", text)]
+        py_msgs = re.findall(r'print\("([^"]*)"\)', inspect.getsource(getattr(legacy, name)))
+        assert m_msgs == py_msgs, name
+        assert "pl_multi_subsvd_call" in text
+    call = open(os.path.join(mdir, "pl_multi_subsvd_call.m")).read()
+    assert "calllib('libparalanczos', 'pl_multi_subsvd'" in call
