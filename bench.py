@@ -109,6 +109,9 @@ def main():
                     help="bound on the CPU baseline's work")
     ap.add_argument("--profile-dominant", action="store_true",
                     help="only run the dominant kernel's launches (for rocprof counter runs)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only copy this many MiB device to device and exit (calibrates the "
+                         "profiler's byte counters against a transfer of known size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the path runs "
@@ -121,6 +124,14 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 4}), flush=True)
+        return 0
     t_setup = time.perf_counter()
     phi = solver.gen_phi("dct", SEED_PHI, M, N)
     G = solver.gram(phi)
