@@ -654,7 +654,7 @@ __device__ __forceinline__ double fmaT(double a, double b, double c) { return __
 template <typename T>
 struct WideShared {
   float lv[WIDE], lw[WIDE];
-  T lh[WIDE];  // Gram-Schmidt coefficients (FP32, or FP64 coefficients)
+  float lh[WIDE];  // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
   float red[8];
   double redd[8];
   int ired[4];
@@ -722,8 +722,8 @@ __device__ __forceinline__ float lds_dot(const float *L, const float (&x)[N]) {
 }
 
 // Gram-Schmidt pass against the LDS basis: lane l forms h_l = v_l . w from
-// column l of V, lane i subtracts sum_l V[i][l] h_l from row i; coefficients
-// and the update sum are accumulated in T
+// column l of V (accumulated in T), lane i subtracts sum_l V[i][l] h_l from
+// row i (FP32)
 template <int S, typename T>
 __device__ __forceinline__ float wide_cgs(float w, const float *V, WideShared<T> &sh) {
   const int gl = threadIdx.x;
@@ -738,12 +738,12 @@ __device__ __forceinline__ float wide_cgs(float w, const float *V, WideShared<T>
     acc0 = fmaT((T)V[(4 * q + 2) * S + gl], (T)t.z, acc0);
     acc1 = fmaT((T)V[(4 * q + 3) * S + gl], (T)t.w, acc1);
   }
-  sh.lh[gl] = acc0 + acc1;
+  sh.lh[gl] = (float)(acc0 + acc1);
   __syncthreads();
-  acc0 = 0;
-  acc1 = 0;
+  float u0 = 0.f, u1 = 0.f;
 #pragma unroll 8
   for (int q = 0; q < WIDE / 4; ++q) {
+    const float4 t = reinterpret_cast<const float4 *>(sh.lh)[q];
     float4 r;
     if constexpr (S % 4 == 0) {
       r = reinterpret_cast<const float4 *>(V + gl * S)[q];
@@ -751,12 +751,12 @@ __device__ __forceinline__ float wide_cgs(float w, const float *V, WideShared<T>
       r = make_float4(V[gl * S + 4 * q], V[gl * S + 4 * q + 1], V[gl * S + 4 * q + 2],
                       V[gl * S + 4 * q + 3]);
     }
-    acc0 = fmaT((T)r.x, sh.lh[4 * q + 0], acc0);
-    acc1 = fmaT((T)r.y, sh.lh[4 * q + 1], acc1);
-    acc0 = fmaT((T)r.z, sh.lh[4 * q + 2], acc0);
-    acc1 = fmaT((T)r.w, sh.lh[4 * q + 3], acc1);
+    u0 = __builtin_fmaf(r.x, t.x, u0);
+    u1 = __builtin_fmaf(r.y, t.y, u1);
+    u0 = __builtin_fmaf(r.z, t.z, u0);
+    u1 = __builtin_fmaf(r.w, t.w, u1);
   }
-  return (float)((T)w - (acc0 + acc1));
+  return w - (u0 + u1);
 }
 
 template <int PAD, int MODE, typename T>
@@ -943,29 +943,39 @@ __global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
 // of row i as vr[m] on lane (i, h). Gram-Schmidt coefficients of the LDS
 // columns are column dot products (lane t forms column t % 256's over the
 // rows of half t / 256); those of the register columns come from a wavefront
-// reduce-scatter of the lanes' products (wave_rs16) and a sum over the four
-// wavefronts of each half in LDS. Coefficients, and the update they feed, are
-// accumulated in T: float, or double (FP64 reorthogonalisation coefficients).
+// reduce-scatter of the lanes' products (wave_rs) and a sum over the four
+// wavefronts of each half in LDS. The coefficients are accumulated in T
+// (float, or double: FP64 reorthogonalisation coefficients) and applied in
+// an FP32 update.
 // ---------------------------------------------------------------------------
-constexpr int K2 = 256;                      // subset capacity
-constexpr int K2T = 512;                     // threads per workgroup
-constexpr int K2L = 140;                     // basis columns in LDS
-constexpr int K2S = 140;                     // LDS row stride: 16-byte rows, stride/4 odd
-constexpr int K2R = (K2 - K2L) / 2;          // register basis columns per lane
-constexpr int K2C = 16;                      // reduce-scatter chunk
-constexpr int K2NC = (K2R + K2C - 1) / K2C;  // chunks
-constexpr int K2H = 72;                      // LDS columns updated by half 0 (rest: half 1)
-constexpr int DPP_ROW_ROR8 = 0x128;          // lane i <- lane (i + 8) mod 16 within a row
+constexpr int K2 = 256;              // subset capacity
+constexpr int K2T = 512;             // threads per workgroup
+constexpr int K2L = 140;             // basis columns in LDS
+constexpr int K2S = 140;             // LDS row stride: 16-byte rows, stride/4 odd
+constexpr int K2R = (K2 - K2L) / 2;  // register basis columns per lane
+constexpr int K2H = 72;              // LDS columns updated by half 0 (rest: half 1)
+constexpr int DPP_ROW_ROR8 = 0x128;  // lane i <- lane (i + 8) mod 16 within a row
+
+// reduce-scatter chunk: 16 FP32 or 8 FP64 values (register budget)
+template <typename T>
+constexpr int k2_chunk() {
+  return sizeof(T) == 8 ? 8 : 16;
+}
+constexpr int K2RP = 64;  // register-column partial slots per (half, wavefront)
+static_assert(K2R <= K2RP, "register partial slots");
 
 template <typename T>
 struct K2Shared {
-  float lv[K2];      // v_j broadcast
+  float lvn[K2];     // next Lanczos vector, unnormalised: the matvec input
   float lw[K2];      // w broadcast (also the index staging of the gather)
   float pw[2][K2];   // matvec partials of the two halves
-  T part[2][K2];     // Gram-Schmidt update partials of the two halves
-  T lh[K2];          // Gram-Schmidt coefficients
-  T cpart[K2];       // LDS-column dot partials of half 1
-  T rpart[2][4][K2NC * K2C];  // register-column partials per half and wavefront
+  float part[2][K2];  // Gram-Schmidt update partials of the two halves
+  float lh[K2];       // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
+  T cpart[K2];        // LDS-column dot partials of half 1
+  T rpart[2][4][K2RP];  // register-column partials per half and wavefront
+  float ra[16];      // alpha partials (p, e) of the eight wavefronts
+  float rb[8];       // beta^2 partials (p, e) of the four wavefronts of half 0
+  float rp[4];       // ||w||^2 partials before a Gram-Schmidt pass
   float red[16];
   double redd[16];
   int ired[8];
@@ -1012,59 +1022,39 @@ __device__ __forceinline__ double dpp(double x) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// Reduce-scatter of 16 values over the 64 lanes of a wavefront: lane q
-// returns sum over all lanes of x[q >> 2] (lanes 4e..4e+3 all hold element
-// e). Partners by stage: lane ^ 32, ^ 16 (permlane swaps), ^ 8 (row_ror:8),
-// ^ 7 (row_half_mirror), then ^ 2 and ^ 1 (quad_perm) as an all-reduce.
-template <typename T>
-__device__ __forceinline__ T wave_rs16(const T (&x)[16], int q) {
-  T y[8], z[4], v[2];
+// Reduce-scatter of N (16 or 8) values over the 64 lanes of a wavefront:
+// lane q returns the sum over all lanes of x[q >> S] (S = 2 for N = 16,
+// 3 for N = 8; 64/N lanes hold each element). Partners by stage: lane ^ 32
+// and ^ 16 (permlane swaps), ^ 8 (row_ror:8), ^ 7 (row_half_mirror), then an
+// all-reduce over ^ 2 and ^ 1 (quad_perm) - the partner sets span all 64 lanes.
+template <int N, typename T>
+__device__ __forceinline__ T wave_rs(const T (&x)[N], int q) {
+  T y[N / 2], z[N / 4];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) y[u] = swap_add<T>(x[u], x[u + 8], true);
+  for (int u = 0; u < N / 2; ++u) y[u] = swap_add<T>(x[u], x[u + N / 2], true);
 #pragma unroll
-  for (int u = 0; u < 4; ++u) z[u] = swap_add<T>(y[u], y[u + 4], false);
-  const bool b3 = (q >> 3) & 1, b2 = (q >> 2) & 1;
+  for (int u = 0; u < N / 4; ++u) z[u] = swap_add<T>(y[u], y[u + N / 4], false);
+  const bool b3 = (q >> 3) & 1;
+  T r;
+  if constexpr (N == 16) {
+    const bool b2 = (q >> 2) & 1;
+    T v[2];
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    const T send = b3 ? z[u] : z[u + 2];
-    const T keep = b3 ? z[u + 2] : z[u];
-    v[u] = keep + dpp<DPP_ROW_ROR8>(send);
+    for (int u = 0; u < 2; ++u) {
+      const T send = b3 ? z[u] : z[u + 2];
+      const T keep = b3 ? z[u + 2] : z[u];
+      v[u] = keep + dpp<DPP_ROW_ROR8>(send);
+    }
+    const T send = b2 ? v[0] : v[1];
+    r = (b2 ? v[1] : v[0]) + dpp<DPP_ROW_HMIRROR>(send);
+  } else {
+    const T send = b3 ? z[0] : z[1];
+    r = (b3 ? z[1] : z[0]) + dpp<DPP_ROW_ROR8>(send);
+    r += dpp<DPP_ROW_HMIRROR>(r);
   }
-  const T send = b2 ? v[0] : v[1];
-  T r = (b2 ? v[1] : v[0]) + dpp<DPP_ROW_HMIRROR>(send);
   r += dpp<DPP_QUAD_2301>(r);
   r += dpp<DPP_QUAD_1032>(r);
   return r;
-}
-
-// compensated dot over the 256 rows (the four wavefronts of half 0; half 1
-// holds the same rows and takes the same result)
-template <typename T>
-__device__ __forceinline__ float k2_dot2(float x, float y, K2Shared<T> &sh) {
-  float p, e;
-  g_dot2_pe<64>(x, y, p, e);
-  const int t = threadIdx.x;
-  __syncthreads();
-  if (t < K2 && (t & 63) == 0) {
-    sh.red[2 * (t >> 6)] = p;
-    sh.red[2 * (t >> 6) + 1] = e;
-  }
-  __syncthreads();
-  float p0 = sh.red[0], e0 = sh.red[1];
-  comp_step(p0, e0, sh.red[2], sh.red[3]);
-  comp_step(p0, e0, sh.red[4], sh.red[5]);
-  comp_step(p0, e0, sh.red[6], sh.red[7]);
-  return p0 + e0;
-}
-
-template <typename T>
-__device__ __forceinline__ float k2_sum(float x, K2Shared<T> &sh) {
-  x = g_sum_dpp<64>(x);
-  const int t = threadIdx.x;
-  __syncthreads();
-  if (t < K2 && (t & 63) == 0) sh.red[8 + (t >> 6)] = x;
-  __syncthreads();
-  return ((sh.red[8] + sh.red[9]) + sh.red[10]) + sh.red[11];
 }
 
 template <bool MAX, typename T>
@@ -1094,45 +1084,87 @@ __device__ __forceinline__ float k2_rowdot(const float (&g)[128], const float *L
   return (a0 + a1) + (a2 + a3);
 }
 
+// beta^2 = w.w (compensated, over the rows of half 0) and the next matvec
+// input lvn = w, published together behind one barrier
+template <typename T>
+__device__ __forceinline__ float k2_beta2(float w, K2Shared<T> &sh) {
+  const int t = threadIdx.x, i = t & (K2 - 1);
+  float p, e;
+  g_dot2_pe<64>(w, w, p, e);
+  if (t < K2) {
+    sh.lvn[i] = w;
+    if ((t & 63) == 0) {
+      sh.rb[2 * (t >> 6)] = p;
+      sh.rb[2 * (t >> 6) + 1] = e;
+    }
+  }
+  __syncthreads();
+  float p0 = sh.rb[0], e0 = sh.rb[1];
+  comp_step(p0, e0, sh.rb[2], sh.rb[3]);
+  comp_step(p0, e0, sh.rb[4], sh.rb[5]);
+  comp_step(p0, e0, sh.rb[6], sh.rb[7]);
+  return p0 + e0;
+}
 
 // One classical Gram-Schmidt pass of w (row i's entry, identical on both
-// halves) against basis columns 0..j; returns the new entry.
-template <typename T>
+// halves) against basis columns 0..j; returns the new entry. With WANT_PRE,
+// ||w||^2 before the pass is returned in pre (its partials ride along with
+// the broadcast of w).
+template <typename T, bool WANT_PRE>
 __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
-                                        const float (&vr)[K2R], K2Shared<T> &sh) {
+                                        const float (&vr)[K2R], K2Shared<T> &sh, float &pre) {
+  constexpr int NC = k2_chunk<T>();
+  constexpr int SH = NC == 16 ? 2 : 3;
   const int t = threadIdx.x, i = t & (K2 - 1), h = t >> 8, wv = (t >> 6) & 3, q = t & 63;
-  if (h == 0) sh.lw[i] = w;
+  float ws = 0.f;
+  if constexpr (WANT_PRE) ws = g_sum_dpp<64>(w * w);
+  if (h == 0) {
+    sh.lw[i] = w;
+    if (WANT_PRE && q == 0) sh.rp[wv] = ws;
+  }
   __syncthreads();
+  if constexpr (WANT_PRE) pre = ((sh.rp[0] + sh.rp[1]) + sh.rp[2]) + sh.rp[3];
   // (1) LDS columns: column i over the rows of half h (rows >= k are zero)
   const int ncl = min(j + 1, K2L);
   T colp = 0;
   if (i < ncl) {
+    // 16 rows per iteration: their column entries are all loaded before the
+    // FMAs consume them, so each iteration pays the LDS latency once
     const int r0 = 128 * h;
-    const int rn = (min(128, max(0, k - r0)) + 7) & ~7;
-    T a0 = 0, a1 = 0;
-    for (int r = 0; r < rn; r += 8) {
+    const int rn = (min(128, max(0, k - r0)) + 15) & ~15;
+    T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int r = 0; r < rn; r += 16) {
+      const float *c = VL + (r0 + r) * K2S + i;
+      float v[16];
 #pragma unroll
-      for (int u = 0; u < 8; u += 2) {
-        a0 = fmaT((T)VL[(r0 + r + u) * K2S + i], (T)sh.lw[r0 + r + u], a0);
-        a1 = fmaT((T)VL[(r0 + r + u + 1) * K2S + i], (T)sh.lw[r0 + r + u + 1], a1);
+      for (int u = 0; u < 16; ++u) v[u] = c[u * K2S];
+      const float4 *lw4 = reinterpret_cast<const float4 *>(sh.lw + r0 + r);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float4 l = lw4[u];
+        a0 = fmaT((T)v[4 * u + 0], (T)l.x, a0);
+        a1 = fmaT((T)v[4 * u + 1], (T)l.y, a1);
+        a2 = fmaT((T)v[4 * u + 2], (T)l.z, a2);
+        a3 = fmaT((T)v[4 * u + 3], (T)l.w, a3);
       }
     }
-    colp = a0 + a1;
+    colp = (a0 + a1) + (a2 + a3);
     if (h == 1) sh.cpart[i] = colp;
   }
-  // (2) register columns K2L + 2m + h, m < nm, by reduce-scatter per chunk
+  // (2) register columns K2L + 2m + h (m < nm): products reduce-scattered per
+  // chunk; columns not yet formed hold zeros, so every chunk runs unguarded
   const int nm = (max(0, j + 1 - K2L) + 1) >> 1;
+  if (nm > 0) {
 #pragma unroll
-  for (int c = 0; c < K2NC; ++c) {
-    if (K2C * c < nm) {
-      T x[K2C];
+    for (int c = 0; c < (K2R + NC - 1) / NC; ++c) {
+      T x[NC];
 #pragma unroll
-      for (int u = 0; u < K2C; ++u) {
-        const int m = K2C * c + u;
+      for (int u = 0; u < NC; ++u) {
+        const int m = NC * c + u;
         x[u] = (m < K2R) ? (T)vr[m < K2R ? m : 0] * (T)w : (T)0;
       }
-      const T r = wave_rs16<T>(x, q);
-      if ((q & 3) == 0) sh.rpart[h][wv][K2C * c + (q >> 2)] = r;
+      const T r = wave_rs<NC, T>(x, q);
+      if ((q & ((1 << SH) - 1)) == 0) sh.rpart[h][wv][NC * c + (q >> SH)] = r;
     }
   }
   __syncthreads();
@@ -1145,31 +1177,49 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
       const int m = (i - K2L) >> 1, hh = (i - K2L) & 1;
       hc = ((sh.rpart[hh][0][m] + sh.rpart[hh][1][m]) + sh.rpart[hh][2][m]) + sh.rpart[hh][3][m];
     }
-    sh.lh[i] = hc;
+    sh.lh[i] = (float)hc;
   }
   __syncthreads();
-  // (4) update partials: LDS columns [0, K2H) by half 0 and [K2H, K2L) by
-  // half 1, register columns by their owner half
-  T u0 = 0, u1 = 0;
+  // (4) update partials (FP32): LDS columns [0, K2H) by half 0 and
+  // [K2H, K2L) by half 1, register columns by their owner half
+  float u0 = 0.f, u1 = 0.f;
   {
     const int c0 = h == 0 ? 0 : K2H;
     const int cn = h == 0 ? min(ncl, K2H) : max(0, ncl - K2H);
     const float *row = VL + i * K2S + c0;
-    const T *hl = sh.lh + c0;
-    for (int c = 0; c < cn; c += 4) {
-      const float4 r = *reinterpret_cast<const float4 *>(row + c);
-      u0 = fmaT((T)r.x, hl[c + 0], u0);
-      u1 = fmaT((T)r.y, hl[c + 1], u1);
-      u0 = fmaT((T)r.z, hl[c + 2], u0);
-      u1 = fmaT((T)r.w, hl[c + 3], u1);
+    const float *hl = sh.lh + c0;
+    // 8 columns per iteration, then at most one group of 4; reads stay in
+    // [c0, c0 + cn rounded up to 4), inside this half's share of the row
+    // (columns past j hold zeros and zero coefficients)
+    const int cn4 = (cn + 3) & ~3;
+    int c = 0;
+    for (; c + 8 <= cn4; c += 8) {
+      const float4 ra = *reinterpret_cast<const float4 *>(row + c);
+      const float4 rb = *reinterpret_cast<const float4 *>(row + c + 4);
+      u0 = __builtin_fmaf(ra.x, hl[c + 0], u0);
+      u1 = __builtin_fmaf(ra.y, hl[c + 1], u1);
+      u0 = __builtin_fmaf(ra.z, hl[c + 2], u0);
+      u1 = __builtin_fmaf(ra.w, hl[c + 3], u1);
+      u0 = __builtin_fmaf(rb.x, hl[c + 4], u0);
+      u1 = __builtin_fmaf(rb.y, hl[c + 5], u1);
+      u0 = __builtin_fmaf(rb.z, hl[c + 6], u0);
+      u1 = __builtin_fmaf(rb.w, hl[c + 7], u1);
+    }
+    if (c < cn4) {
+      const float4 ra = *reinterpret_cast<const float4 *>(row + c);
+      u0 = __builtin_fmaf(ra.x, hl[c + 0], u0);
+      u1 = __builtin_fmaf(ra.y, hl[c + 1], u1);
+      u0 = __builtin_fmaf(ra.z, hl[c + 2], u0);
+      u1 = __builtin_fmaf(ra.w, hl[c + 3], u1);
     }
   }
+  if (nm > 0) {
 #pragma unroll
-  for (int m = 0; m < K2R; ++m)
-    if (m < nm) u0 = fmaT((T)vr[m], sh.lh[K2L + 2 * m + h], u0);
+    for (int m = 0; m < K2R; ++m) u0 = __builtin_fmaf(vr[m], sh.lh[K2L + 2 * m + h], u0);
+  }
   sh.part[h][i] = u0 + u1;
   __syncthreads();
-  return (float)((T)w - (sh.part[0][i] + sh.part[1][i]));
+  return w - (sh.part[0][i] + sh.part[1][i]);
 }
 
 template <int MODE, typename T>
@@ -1239,36 +1289,54 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
     constexpr uint32_t FATAL = ST_BADK | ST_BADINDEX | ST_NONFINITE;
     if (!(st & FATAL)) {
       // ---- (b) Lanczos recurrence, ParaLanczos.cu:1285-1374 with the build's
-      // substitutions (ones/sqrt(k) start, full reorthogonalisation, restart) ----
+      // substitutions (ones/sqrt(k) start, full reorthogonalisation, restart).
+      // Each step: matvec of the unnormalised vector published in lvn, scaled
+      // by 1/beta, with alpha's compensated partials exchanged alongside;
+      // three-term update; Gram-Schmidt pass (||w||^2 rides along); beta^2
+      // published with the next vector. Six barriers per step. ----
       float vr[K2R];
 #pragma unroll
       for (int m = 0; m < K2R; ++m) vr[m] = 0.f;
       for (int e = t; e < K2 * K2S; e += K2T) VL[e] = 0.f;
-      float vi = (i < k) ? (1.0f / sqrtf((float)k)) : 0.f;
+      float scale = 1.0f / sqrtf((float)k);
+      float vi = (i < k) ? scale : 0.f;
       __syncthreads();
-      if (h == 0) VL[i * K2S] = vi;
+      if (h == 0) {
+        VL[i * K2S] = vi;
+        sh.lvn[i] = (i < k) ? 1.0f : 0.f;
+      }
+      __syncthreads();
       float vp = 0.f, bj = 0.f, amax = 0.f, bmax = 0.f, my_a = 0.f, my_b = 0.f;
       for (int j = 0; j < k; ++j) {
-        if (h == 0) sh.lv[i] = vi;
+        // matvec partial over this half's columns; alpha's partial Dot2 of v
+        // with it (the two halves' partials sum to v . G lvn)
+        const float mv = k2_rowdot(g, sh.lvn + 128 * h);
+        float p, e;
+        g_dot2_pe<64>(vi, mv, p, e);
+        sh.pw[h][i] = mv;
+        if (q == 0) {
+          sh.ra[2 * (t >> 6)] = p;
+          sh.ra[2 * (t >> 6) + 1] = e;
+        }
         __syncthreads();
-        sh.pw[h][i] = k2_rowdot(g, sh.lv + 128 * h);
-        __syncthreads();
-        float w = sh.pw[0][i] + sh.pw[1][i];
-        const float al = k2_dot2(vi, w, sh);
+        float w = (sh.pw[0][i] + sh.pw[1][i]) * scale;
+        float p0 = sh.ra[0], e0 = sh.ra[1];
+#pragma unroll
+        for (int w2 = 1; w2 < 8; ++w2) comp_step(p0, e0, sh.ra[2 * w2], sh.ra[2 * w2 + 1]);
+        const float al = (p0 + e0) * scale;
         my_a = (i == j) ? al : my_a;
         amax = fmaxf(amax, fabsf(al));
-        w = __builtin_fmaf(-al, vi, __builtin_fmaf(-bj, vp, w));
-        const float pre = k2_sum(w * w, sh);
-        w = k2_cgs<T>(w, j, k, VL, vr, sh);
         if (j == k - 1) break;
-        float b2 = k2_dot2(w, w, sh);
+        w = __builtin_fmaf(-al, vi, __builtin_fmaf(-bj, vp, w));
+        float pre = 0.f, unused;
+        w = k2_cgs<T, true>(w, j, k, VL, vr, sh, pre);
+        float b2 = k2_beta2(w, sh);
         if (b2 < 0.5f * pre) {  // DGKS: w was mostly in span(V); orthogonalise again
-          w = k2_cgs<T>(w, j, k, VL, vr, sh);
-          b2 = k2_dot2(w, w, sh);
+          w = k2_cgs<T, false>(w, j, k, VL, vr, sh, unused);
+          b2 = k2_beta2(w, sh);
         }
         const float b = sqrtf(b2);
         vp = vi;
-        float scale;
         if (b > TAU * (amax + bmax)) {
           bj = b;
           bmax = fmaxf(bmax, b);
@@ -1289,7 +1357,6 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
           const float res = 1.f - (sh.pw[0][i] + sh.pw[1][i]);
           const int wb = restart_lane<64>(res, i, k);
           const float wvv = __shfl(i < k ? res : -1.f, wb & 63, 64);
-          __syncthreads();
           if (t < K2 && q == 0) {
             sh.red[t >> 6] = wvv;
             sh.ired[t >> 6] = wb;
@@ -1303,9 +1370,9 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
               bi = sh.ired[w2];
             }
           w = (i == bi) ? 1.f : 0.f;
-          w = k2_cgs<T>(w, j, k, VL, vr, sh);
-          w = k2_cgs<T>(w, j, k, VL, vr, sh);
-          scale = 1.0f / sqrtf(k2_dot2(w, w, sh));
+          w = k2_cgs<T, false>(w, j, k, VL, vr, sh, unused);
+          w = k2_cgs<T, false>(w, j, k, VL, vr, sh, unused);
+          scale = 1.0f / sqrtf(k2_beta2(w, sh));
         }
         vi = w * scale;
         // basis column j + 1: LDS, or register slot (j + 1 - K2L) / 2 of one half
