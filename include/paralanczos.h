@@ -136,12 +136,14 @@ pl_status pl_plan_destroy(pl_plan *plan);
 /*
  * Tuning table of PL_FORM_TUNED: the variant (lanes per subset, LDS row
  * padding) used for each k; mixed batches are binned by runs of equal
- * variants. pl_autotune times every variant with lanes >= k (lane groups of
- * 8..128, which give bit-identical results, and the 512-lane kernel beyond
- * k = 128) and pad 0/1/4 on
+ * variants. pl_autotune times every variant with lanes >= k of the family
+ * that first covers k (lane groups of 8..64 lanes up to k = 64, the 128-lane
+ * kernel up to 128, the 512-lane kernel beyond; a family gives bit-identical
+ * results whatever its lanes and padding) and pad 0/1/4 on
  * `sample` seeded subsets of each size in klist (nk sizes, each <= n), keeps
  * the fastest, and gives every unlisted size the winner of the next listed
- * size above it. best_ms (nk entries, may be NULL) receives the winning
+ * size above it when that winner is of the size's own family (else the
+ * default). best_ms (nk entries, may be NULL) receives the winning
  * times. Synchronous; the table is shared by all threads of the process.
  */
 pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klist, int nk,

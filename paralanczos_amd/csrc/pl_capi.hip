@@ -530,9 +530,11 @@ pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klis
     Variant best{0, 0};
     float best_t = 0.f;
     for (int lanes : kLanes) {
-      // lane groups up to 128 compute bit-identical results; the 512-lane
-      // kernel (its own reduction order) is the one choice beyond k = 128
-      if (lanes < k || (lanes == 256 && k <= 128)) continue;
+      // the lane groups of 8..64 lanes compute bit-identical results, as do
+      // the 128-lane paddings; the 128- and 512-lane kernels (each with its
+      // own summation order) are offered only where the smaller family cannot
+      // take k, so tuning never changes a result
+      if (lanes < k || (lanes == 128 && k <= 64) || (lanes == 256 && k <= 128)) continue;
       for (int pad : kPads) {
         if (!pl::solve_variant_exists(lanes, pad)) continue;
         SolveJob j{G,   n, ld, idx, nullptr, k, sample, out, out + sample, out + 2 * sample,
@@ -568,15 +570,19 @@ pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klis
   }
   if (st == PL_OK) {
     // bin edges: a size between two measured ones takes the winner of the
-    // next measured size above it (whose lane count covers it)
+    // next measured size above it (whose lane count covers it) when that
+    // winner belongs to the size's own kernel family, else keeps its default
     std::vector<int> ks(klist, klist + nk);
     std::sort(ks.begin(), ks.end());
     ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+    auto family = [](int lanes) { return lanes < 128 ? 0 : lanes == 128 ? 1 : 2; };
     std::lock_guard<std::mutex> lock(g_tune_mu);
     size_t q = 0;
     for (int k = 1; k <= ks.back(); ++k) {
       while (ks[q] < k) ++q;
-      g_tune[k] = g_tune[ks[q]];
+      if (ks[q] == k) continue;
+      const Variant w = g_tune[ks[q]];
+      g_tune[k] = family(w.lanes) == family(default_variant(k).lanes) ? w : default_variant(k);
     }
   }
   if (e0) (void)hipEventDestroy(e0);

@@ -653,39 +653,16 @@ __device__ __forceinline__ double fmaT(double a, double b, double c) { return __
 
 template <typename T>
 struct WideShared {
-  float lv[WIDE], lw[WIDE];
+  float lv[WIDE];  // next Lanczos vector, unnormalised: the matvec input
+  float lw[WIDE];  // w broadcast
   float lh[WIDE];  // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
+  float ra[4];     // alpha partials (p, e) of the two wavefronts
+  float rb[4];     // beta^2 partials
+  float rp[2];     // ||w||^2 partials before a Gram-Schmidt pass
   float red[8];
   double redd[8];
   int ired[4];
 };
-
-// compensated dot over the workgroup: per-wave Dot2, then the two wave
-// partials combined in wave order (every lane gets identical bits)
-template <typename T>
-__device__ __forceinline__ float wide_dot2(float x, float y, WideShared<T> &sh) {
-  float p, e;
-  g_dot2_pe<64>(x, y, p, e);
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) {
-    sh.red[2 * w] = p;
-    sh.red[2 * w + 1] = e;
-  }
-  __syncthreads();
-  float p0 = sh.red[0], e0 = sh.red[1];
-  comp_step(p0, e0, sh.red[2], sh.red[3]);
-  return p0 + e0;
-}
-
-template <typename T>
-__device__ __forceinline__ float wide_sum(float x, WideShared<T> &sh) {
-  x = g_sum_dpp<64>(x);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) sh.red[4 + (threadIdx.x >> 6)] = x;
-  __syncthreads();
-  return sh.red[4] + sh.red[5];
-}
 
 template <typename T>
 __device__ __forceinline__ double wide_min(double x, WideShared<T> &sh) {
@@ -705,58 +682,96 @@ __device__ __forceinline__ double wide_max(double x, WideShared<T> &sh) {
   return fmax(sh.redd[2], sh.redd[3]);
 }
 
-// sum_c x[c] * L[c] over a 16-byte aligned LDS vector read by broadcast
-// (two accumulators split by index parity)
+// sum_c x[c] * L[c] over a 16-byte aligned LDS vector read by broadcast;
+// four accumulators by index mod 4
 template <int N>
-__device__ __forceinline__ float lds_dot(const float *L, const float (&x)[N]) {
-  float acc0 = 0.f, acc1 = 0.f;
+__device__ __forceinline__ float lds_dot4(const float *L, const float (&x)[N]) {
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
 #pragma unroll
   for (int q = 0; q < N / 4; ++q) {
     const float4 t = reinterpret_cast<const float4 *>(L)[q];
-    acc0 = __builtin_fmaf(x[4 * q + 0], t.x, acc0);
-    acc1 = __builtin_fmaf(x[4 * q + 1], t.y, acc1);
-    acc0 = __builtin_fmaf(x[4 * q + 2], t.z, acc0);
-    acc1 = __builtin_fmaf(x[4 * q + 3], t.w, acc1);
+    a0 = __builtin_fmaf(x[4 * q + 0], t.x, a0);
+    a1 = __builtin_fmaf(x[4 * q + 1], t.y, a1);
+    a2 = __builtin_fmaf(x[4 * q + 2], t.z, a2);
+    a3 = __builtin_fmaf(x[4 * q + 3], t.w, a3);
   }
-  return acc0 + acc1;
+  return (a0 + a1) + (a2 + a3);
 }
 
-// Gram-Schmidt pass against the LDS basis: lane l forms h_l = v_l . w from
-// column l of V (accumulated in T), lane i subtracts sum_l V[i][l] h_l from
-// row i (FP32)
-template <int S, typename T>
-__device__ __forceinline__ float wide_cgs(float w, const float *V, WideShared<T> &sh) {
+// beta^2 = w.w (compensated: per-wave Dot2, the two partials combined in
+// wave order) and the next matvec input lv = w, behind one barrier
+template <typename T>
+__device__ __forceinline__ float wide_beta2(float w, WideShared<T> &sh) {
   const int gl = threadIdx.x;
-  sh.lw[gl] = w;
-  __syncthreads();
-  T acc0 = 0, acc1 = 0;
-#pragma unroll 8
-  for (int q = 0; q < WIDE / 4; ++q) {
-    const float4 t = reinterpret_cast<const float4 *>(sh.lw)[q];
-    acc0 = fmaT((T)V[(4 * q + 0) * S + gl], (T)t.x, acc0);
-    acc1 = fmaT((T)V[(4 * q + 1) * S + gl], (T)t.y, acc1);
-    acc0 = fmaT((T)V[(4 * q + 2) * S + gl], (T)t.z, acc0);
-    acc1 = fmaT((T)V[(4 * q + 3) * S + gl], (T)t.w, acc1);
+  float p, e;
+  g_dot2_pe<64>(w, w, p, e);
+  sh.lv[gl] = w;
+  if ((gl & 63) == 0) {
+    sh.rb[2 * (gl >> 6)] = p;
+    sh.rb[2 * (gl >> 6) + 1] = e;
   }
-  sh.lh[gl] = (float)(acc0 + acc1);
   __syncthreads();
-  float u0 = 0.f, u1 = 0.f;
-#pragma unroll 8
-  for (int q = 0; q < WIDE / 4; ++q) {
-    const float4 t = reinterpret_cast<const float4 *>(sh.lh)[q];
+  float p0 = sh.rb[0], e0 = sh.rb[1];
+  comp_step(p0, e0, sh.rb[2], sh.rb[3]);
+  return p0 + e0;
+}
+
+// Gram-Schmidt pass against basis columns 0..j in LDS: lane l forms
+// h_l = v_l . w from column l over the first k rows (accumulated in T), lane
+// i subtracts sum_l V[i][l] h_l from row i (FP32). With WANT_PRE, ||w||^2
+// before the pass comes back in pre (its partials ride with the broadcast).
+template <int S, typename T, bool WANT_PRE>
+__device__ __forceinline__ float wide_cgs(float w, int j, int k, const float *V,
+                                          WideShared<T> &sh, float &pre) {
+  const int gl = threadIdx.x;
+  float ws = 0.f;
+  if constexpr (WANT_PRE) ws = g_sum_dpp<64>(w * w);
+  sh.lw[gl] = w;
+  if (WANT_PRE && (gl & 63) == 0) sh.rp[gl >> 6] = ws;
+  __syncthreads();
+  if constexpr (WANT_PRE) pre = sh.rp[0] + sh.rp[1];
+  const int ncol = j + 1;
+  T hc = 0;
+  if (gl < ncol) {
+    // 16 rows per iteration, loaded before the FMAs consume them; rows >= k
+    // are zero (k <= 128, so the rounded count stays inside the tile)
+    const int rn = (k + 15) & ~15;
+    T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int r = 0; r < rn; r += 16) {
+      float v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = V[(r + u) * S + gl];
+      const float4 *lw4 = reinterpret_cast<const float4 *>(sh.lw + r);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float4 l = lw4[u];
+        a0 = fmaT((T)v[4 * u + 0], (T)l.x, a0);
+        a1 = fmaT((T)v[4 * u + 1], (T)l.y, a1);
+        a2 = fmaT((T)v[4 * u + 2], (T)l.z, a2);
+        a3 = fmaT((T)v[4 * u + 3], (T)l.w, a3);
+      }
+    }
+    hc = (a0 + a1) + (a2 + a3);
+  }
+  sh.lh[gl] = (float)hc;  // zero for the columns not yet formed
+  __syncthreads();
+  // row update over columns 0..j rounded up to 4 (the rest hold zeros)
+  const int nc4 = (ncol + 3) & ~3;
+  float u0 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f;
+  for (int c = 0; c < nc4; c += 4) {
+    const float4 t = *reinterpret_cast<const float4 *>(sh.lh + c);
     float4 r;
     if constexpr (S % 4 == 0) {
-      r = reinterpret_cast<const float4 *>(V + gl * S)[q];
+      r = *reinterpret_cast<const float4 *>(V + gl * S + c);
     } else {
-      r = make_float4(V[gl * S + 4 * q], V[gl * S + 4 * q + 1], V[gl * S + 4 * q + 2],
-                      V[gl * S + 4 * q + 3]);
+      r = make_float4(V[gl * S + c], V[gl * S + c + 1], V[gl * S + c + 2], V[gl * S + c + 3]);
     }
     u0 = __builtin_fmaf(r.x, t.x, u0);
     u1 = __builtin_fmaf(r.y, t.y, u1);
-    u0 = __builtin_fmaf(r.z, t.z, u0);
-    u1 = __builtin_fmaf(r.w, t.w, u1);
+    u2 = __builtin_fmaf(r.z, t.z, u2);
+    u3 = __builtin_fmaf(r.w, t.w, u3);
   }
-  return w - (u0 + u1);
+  return w - ((u0 + u1) + (u2 + u3));
 }
 
 template <int PAD, int MODE, typename T>
@@ -837,30 +852,44 @@ __device__ __forceinline__ void solve_wide_subset(const SolveArgs &a, float *V, 
     constexpr uint32_t FATAL = ST_BADK | ST_BADINDEX | ST_NONFINITE;
     if (!(st & FATAL)) {
       // ---- (b) Lanczos recurrence, ParaLanczos.cu:1285-1374 with the build's
-      // substitutions (ones/sqrt(k) start, full reorthogonalisation, restart) ----
-      float vi = (gl < k) ? (1.0f / sqrtf((float)k)) : 0.f;
+      // substitutions (ones/sqrt(k) start, full reorthogonalisation, restart).
+      // Each step: matvec of the unnormalised vector in lv, scaled by 1/beta,
+      // alpha's compensated partials exchanged behind the first barrier;
+      // three-term update; Gram-Schmidt pass (||w||^2 rides along, two
+      // barriers); beta^2 published with the next vector (one barrier). ----
+      float scale = 1.0f / sqrtf((float)k);
+      float vi = (gl < k) ? scale : 0.f;
 #pragma unroll
       for (int c = 0; c < WIDE; ++c) V[gl * S + c] = (c == 0) ? vi : 0.f;
+      sh.lv[gl] = (gl < k) ? 1.f : 0.f;
+      __syncthreads();
       float vp = 0.f, bj = 0.f, amax = 0.f, bmax = 0.f, my_a = 0.f, my_b = 0.f;
       for (int j = 0; j < k; ++j) {
-        sh.lv[gl] = vi;
+        const float mv = lds_dot4<WIDE>(sh.lv, g);
+        float p, e;
+        g_dot2_pe<64>(vi, mv, p, e);
+        if ((gl & 63) == 0) {
+          sh.ra[2 * (gl >> 6)] = p;
+          sh.ra[2 * (gl >> 6) + 1] = e;
+        }
         __syncthreads();
-        float w = lds_dot<WIDE>(sh.lv, g);
-        const float al = wide_dot2(vi, w, sh);
+        float p0 = sh.ra[0], e0 = sh.ra[1];
+        comp_step(p0, e0, sh.ra[2], sh.ra[3]);
+        const float al = (p0 + e0) * scale;
+        float w = mv * scale;
         my_a = (gl == j) ? al : my_a;
         amax = fmaxf(amax, fabsf(al));
-        w = __builtin_fmaf(-al, vi, __builtin_fmaf(-bj, vp, w));
-        const float pre = wide_sum(w * w, sh);
-        w = wide_cgs<S, T>(w, V, sh);
         if (j == k - 1) break;
-        float b2 = wide_dot2(w, w, sh);
+        w = __builtin_fmaf(-al, vi, __builtin_fmaf(-bj, vp, w));
+        float pre = 0.f, unused;
+        w = wide_cgs<S, T, true>(w, j, k, V, sh, pre);
+        float b2 = wide_beta2(w, sh);
         if (b2 < 0.5f * pre) {  // DGKS: w was mostly in span(V); orthogonalise again
-          w = wide_cgs<S, T>(w, V, sh);
-          b2 = wide_dot2(w, w, sh);
+          w = wide_cgs<S, T, false>(w, j, k, V, sh, unused);
+          b2 = wide_beta2(w, sh);
         }
         const float b = sqrtf(b2);
         vp = vi;
-        float scale;
         if (b > TAU * (amax + bmax)) {
           bj = b;
           bmax = fmaxf(bmax, b);
@@ -882,9 +911,9 @@ __device__ __forceinline__ void solve_wide_subset(const SolveArgs &a, float *V, 
           __syncthreads();
           const int bi = (sh.red[1] > sh.red[0]) ? sh.ired[1] : sh.ired[0];
           w = (gl == bi) ? 1.f : 0.f;
-          w = wide_cgs<S, T>(w, V, sh);
-          w = wide_cgs<S, T>(w, V, sh);
-          scale = 1.0f / sqrtf(wide_dot2(w, w, sh));
+          w = wide_cgs<S, T, false>(w, j, k, V, sh, unused);
+          w = wide_cgs<S, T, false>(w, j, k, V, sh, unused);
+          scale = 1.0f / sqrtf(wide_beta2(w, sh));
         }
         vi = w * scale;
         V[gl * S + j + 1] = vi;
@@ -971,7 +1000,7 @@ struct K2Shared {
   float pw[2][K2];   // matvec partials of the two halves
   float part[2][K2];  // Gram-Schmidt update partials of the two halves
   float lh[K2];       // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
-  T cpart[K2];        // LDS-column dot partials of half 1
+  T cp[2 * K2];       // LDS-column dot partials: row group g, column c at g*C + c
   T rpart[2][4][K2RP];  // register-column partials per half and wavefront
   float ra[16];      // alpha partials (p, e) of the eight wavefronts
   float rb[8];       // beta^2 partials (p, e) of the four wavefronts of half 0
@@ -1124,32 +1153,38 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
   }
   __syncthreads();
   if constexpr (WANT_PRE) pre = ((sh.rp[0] + sh.rp[1]) + sh.rp[2]) + sh.rp[3];
-  // (1) LDS columns: column i over the rows of half h (rows >= k are zero)
+  // (1) LDS columns 0..ncl-1: the 512 lanes split into C columns x 512/C row
+  // groups of C/2 rows (C = 64, 128 or 256, the least that covers ncl), so
+  // every lane works and the dependent chains stay short; rows >= k are zero
   const int ncl = min(j + 1, K2L);
-  T colp = 0;
-  if (i < ncl) {
-    // 16 rows per iteration: their column entries are all loaded before the
-    // FMAs consume them, so each iteration pays the LDS latency once
-    const int r0 = 128 * h;
-    const int rn = (min(128, max(0, k - r0)) + 15) & ~15;
-    T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    for (int r = 0; r < rn; r += 16) {
-      const float *c = VL + (r0 + r) * K2S + i;
-      float v[16];
+  const int lgC = ncl <= 64 ? 6 : ncl <= 128 ? 7 : 8;
+  const int C = 1 << lgC;
+  {
+    const int c = t & (C - 1), g = t >> lgC;
+    if (c < ncl) {
+      // 16 rows per iteration: their column entries are all loaded before the
+      // FMAs consume them, so each iteration pays the LDS latency once
+      const int R = C >> 1;
+      const int r0 = g * R;
+      const int rn = (min(R, max(0, k - r0)) + 15) & ~15;
+      T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+      for (int r = 0; r < rn; r += 16) {
+        const float *col = VL + (r0 + r) * K2S + c;
+        float v[16];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) v[u] = c[u * K2S];
-      const float4 *lw4 = reinterpret_cast<const float4 *>(sh.lw + r0 + r);
+        for (int u = 0; u < 16; ++u) v[u] = col[u * K2S];
+        const float4 *lw4 = reinterpret_cast<const float4 *>(sh.lw + r0 + r);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float4 l = lw4[u];
-        a0 = fmaT((T)v[4 * u + 0], (T)l.x, a0);
-        a1 = fmaT((T)v[4 * u + 1], (T)l.y, a1);
-        a2 = fmaT((T)v[4 * u + 2], (T)l.z, a2);
-        a3 = fmaT((T)v[4 * u + 3], (T)l.w, a3);
+        for (int u = 0; u < 4; ++u) {
+          const float4 l = lw4[u];
+          a0 = fmaT((T)v[4 * u + 0], (T)l.x, a0);
+          a1 = fmaT((T)v[4 * u + 1], (T)l.y, a1);
+          a2 = fmaT((T)v[4 * u + 2], (T)l.z, a2);
+          a3 = fmaT((T)v[4 * u + 3], (T)l.w, a3);
+        }
       }
+      sh.cp[g * C + c] = (a0 + a1) + (a2 + a3);
     }
-    colp = (a0 + a1) + (a2 + a3);
-    if (h == 1) sh.cpart[i] = colp;
   }
   // (2) register columns K2L + 2m + h (m < nm): products reduce-scattered per
   // chunk; columns not yet formed hold zeros, so every chunk runs unguarded
@@ -1168,11 +1203,14 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
     }
   }
   __syncthreads();
-  // (3) the coefficients, lane t < 256 for column t (zero beyond j)
+  // (3) the coefficients, lane t < 256 for column t (zero beyond j); the row
+  // groups' partials are summed in group order
   if (h == 0) {
     T hc = 0;
     if (i < ncl) {
-      hc = colp + sh.cpart[i];
+      const int ng = 512 >> lgC;
+      hc = sh.cp[i];
+      for (int g = 1; g < ng; ++g) hc += sh.cp[g * C + i];
     } else if (i >= K2L && i <= j) {
       const int m = (i - K2L) >> 1, hh = (i - K2L) & 1;
       hc = ((sh.rpart[hh][0][m] + sh.rpart[hh][1][m]) + sh.rpart[hh][2][m]) + sh.rpart[hh][3][m];
@@ -1182,7 +1220,7 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
   __syncthreads();
   // (4) update partials (FP32): LDS columns [0, K2H) by half 0 and
   // [K2H, K2L) by half 1, register columns by their owner half
-  float u0 = 0.f, u1 = 0.f;
+  float u0 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f;
   {
     const int c0 = h == 0 ? 0 : K2H;
     const int cn = h == 0 ? min(ncl, K2H) : max(0, ncl - K2H);
@@ -1198,26 +1236,29 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
       const float4 rb = *reinterpret_cast<const float4 *>(row + c + 4);
       u0 = __builtin_fmaf(ra.x, hl[c + 0], u0);
       u1 = __builtin_fmaf(ra.y, hl[c + 1], u1);
-      u0 = __builtin_fmaf(ra.z, hl[c + 2], u0);
-      u1 = __builtin_fmaf(ra.w, hl[c + 3], u1);
+      u2 = __builtin_fmaf(ra.z, hl[c + 2], u2);
+      u3 = __builtin_fmaf(ra.w, hl[c + 3], u3);
       u0 = __builtin_fmaf(rb.x, hl[c + 4], u0);
       u1 = __builtin_fmaf(rb.y, hl[c + 5], u1);
-      u0 = __builtin_fmaf(rb.z, hl[c + 6], u0);
-      u1 = __builtin_fmaf(rb.w, hl[c + 7], u1);
+      u2 = __builtin_fmaf(rb.z, hl[c + 6], u2);
+      u3 = __builtin_fmaf(rb.w, hl[c + 7], u3);
     }
     if (c < cn4) {
       const float4 ra = *reinterpret_cast<const float4 *>(row + c);
       u0 = __builtin_fmaf(ra.x, hl[c + 0], u0);
       u1 = __builtin_fmaf(ra.y, hl[c + 1], u1);
-      u0 = __builtin_fmaf(ra.z, hl[c + 2], u0);
-      u1 = __builtin_fmaf(ra.w, hl[c + 3], u1);
+      u2 = __builtin_fmaf(ra.z, hl[c + 2], u2);
+      u3 = __builtin_fmaf(ra.w, hl[c + 3], u3);
     }
   }
   if (nm > 0) {
 #pragma unroll
-    for (int m = 0; m < K2R; ++m) u0 = __builtin_fmaf(vr[m], sh.lh[K2L + 2 * m + h], u0);
+    for (int m = 0; m < K2R; m += 2) {
+      u0 = __builtin_fmaf(vr[m], sh.lh[K2L + 2 * m + h], u0);
+      if (m + 1 < K2R) u1 = __builtin_fmaf(vr[m + 1], sh.lh[K2L + 2 * m + 2 + h], u1);
+    }
   }
-  sh.part[h][i] = u0 + u1;
+  sh.part[h][i] = (u0 + u1) + (u2 + u3);
   __syncthreads();
   return w - (sh.part[0][i] + sh.part[1][i]);
 }

@@ -1,11 +1,10 @@
 """Mixed-size batching on the GPU: device-side binning, persistent per-bin
 launches, graph-captured plans and the autotuner.
 
-Every form computes each subset with the same arithmetic (zero padding makes
-the result independent of the lane count and LDS padding), so the tuned
-binned form, the plain one-launch-per-k form, a single explicit variant and
-the graph replays must agree bit for bit; the oracle pins the values
-themselves (relative error <= 1e-5).
+Within a kernel family a subset's arithmetic does not depend on the lane
+count or LDS padding (zero padding), so the tuned binned form, the plain
+one-launch-per-k form and the graph replays agree bit for bit; the oracle
+pins the values themselves (relative error <= 1e-5).
 """
 import numpy as np
 import pytest
@@ -42,14 +41,19 @@ def _np(ts):
 
 def test_tuned_plain_variant_bitexact(mixed):
     """With FP32 Gram-Schmidt coefficients at every k (the tuned form switches
-    to FP64 ones at k = 128 by default) the three forms agree bit for bit."""
+    to FP64 ones at k = 128 by default) the tuned and plain forms agree bit for
+    bit: both use the lane-group family up to k = 64 and the 128-lane kernel
+    beyond. One 128-lane launch for every size agrees with them bit for bit
+    beyond k = 64 and to 1e-5 below (its own summation order)."""
     G, ks, off, idx = mixed
     tuned = _np(solver.solve(G, idx, k=128, offsets=off, form="tuned", reorth="f32"))
     plain = _np(solver.solve(G, idx, k=128, offsets=off, form="plain"))
     single = _np(solver.solve(G, idx, k=128, offsets=off, lanes=128, pad=4))
+    kk0 = ks.cpu().numpy()
     for a, b, c in zip(tuned, plain, single):
         assert np.array_equal(a, b, equal_nan=True)
-        assert np.array_equal(a, c, equal_nan=True)
+        assert np.array_equal(a[kk0 > 64], c[kk0 > 64], equal_nan=True)
+    assert rel(single[0], tuned[0]).max() <= RTOL and rel(single[1], tuned[1]).max() <= RTOL
     assert np.all(tuned[3] == 0)
     # the default tuned form differs only in the k = 128 subsets, within 1e-5
     dflt = _np(solver.solve(G, idx, k=128, offsets=off))
@@ -133,9 +137,16 @@ def test_autotune(mixed):
         for k, (lanes, pad, ms) in res.items():
             assert lanes >= k and pad in (0, 1, 4) and ms > 0
         # unlisted sizes take the winner of the next listed size above them
+        # when it is of their own kernel family (lane groups up to 64 lanes,
+        # then the 128-lane kernel), else their default
+        fam = lambda lanes: 0 if lanes < 128 else 1
         for k in range(1, 73):
             nxt = min(x for x in (8, 24, 40, 72) if x >= k)
-            assert solver.tuning_get(k) == solver.tuning_get(nxt)[:2]
+            want = solver.tuning_get(nxt)
+            dflt = (max(8, 1 << (k - 1).bit_length()), 4)
+            if fam(want[0]) != fam(dflt[0]):
+                want = dflt
+            assert solver.tuning_get(k) == want, k
         after = _np(solver.solve(G, idx, k=128, offsets=off))
         for a, b in zip(before, after):
             assert np.array_equal(a, b, equal_nan=True)

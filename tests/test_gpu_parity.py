@@ -117,21 +117,24 @@ def test_variants_all_k(gpu, k):
     Gh = G.cpu().numpy()
     ih = idx.cpu().numpy()
     olo, ohi, _, ost = oracle.solve_batch(Gh, ih, k=k)
-    ref = None
-    for lanes in (8, 16, 32, 64, 128):
+    refs = {}
+    for lanes in (8, 16, 32, 64, 128, 256):
         if lanes < k:
             continue
-        for pad in ((0, 1, 4) if lanes < 128 else (1, 4)):
+        for pad in ((0, 1, 4) if lanes < 128 else (1, 4) if lanes == 128 else (4,)):
             lo, hi, de, st = [t.cpu().numpy() for t in solver.solve(G, idx, k=k, lanes=lanes, pad=pad)]
             assert np.all((st & ~1) == 0), (lanes, pad, st)
             assert rel(lo, olo).max() <= RTOL, (lanes, pad)
             assert rel(hi, ohi).max() <= RTOL, (lanes, pad)
-            # zero padding keeps the Lanczos arithmetic identical across lane
-            # counts and LDS paddings, and the bisection is per subset: the
-            # outputs are bit-identical in every variant
-            if ref is None:
-                ref = (lo, hi)
+            # zero padding keeps the arithmetic of a kernel family identical
+            # across lane counts and LDS paddings (the lane groups of 8..64
+            # lanes; the 128-lane paddings), and the bisection is per subset:
+            # within a family the outputs are bit-identical
+            fam = "group" if lanes < 128 else lanes
+            if fam not in refs:
+                refs[fam] = (lo, hi)
             else:
+                ref = refs[fam]
                 assert np.array_equal(lo, ref[0]) and np.array_equal(hi, ref[1]), (lanes, pad)
 
 
