@@ -655,6 +655,7 @@ template <typename T>
 struct WideShared {
   float lv[WIDE];  // next Lanczos vector, unnormalised: the matvec input
   float lw[WIDE];  // w broadcast
+  T lwT[sizeof(T) == 8 ? WIDE : 4];  // w in T (FP64 coefficients only)
   float lh[WIDE];  // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
   float ra[4];     // alpha partials (p, e) of the two wavefronts
   float rb[4];     // beta^2 partials
@@ -727,6 +728,7 @@ __device__ __forceinline__ float wide_cgs(float w, int j, int k, const float *V,
   float ws = 0.f;
   if constexpr (WANT_PRE) ws = g_sum_dpp<64>(w * w);
   sh.lw[gl] = w;
+  if constexpr (sizeof(T) == 8) sh.lwT[gl] = (T)w;
   if (WANT_PRE && (gl & 63) == 0) sh.rp[gl >> 6] = ws;
   __syncthreads();
   if constexpr (WANT_PRE) pre = sh.rp[0] + sh.rp[1];
@@ -741,14 +743,26 @@ __device__ __forceinline__ float wide_cgs(float w, int j, int k, const float *V,
       float v[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) v[u] = V[(r + u) * S + gl];
-      const float4 *lw4 = reinterpret_cast<const float4 *>(sh.lw + r);
+      if constexpr (sizeof(T) == 8) {
+        const double2 *lw2 = reinterpret_cast<const double2 *>(sh.lwT + r);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float4 l = lw4[u];
-        a0 = fmaT((T)v[4 * u + 0], (T)l.x, a0);
-        a1 = fmaT((T)v[4 * u + 1], (T)l.y, a1);
-        a2 = fmaT((T)v[4 * u + 2], (T)l.z, a2);
-        a3 = fmaT((T)v[4 * u + 3], (T)l.w, a3);
+        for (int u = 0; u < 4; ++u) {
+          const double2 l0 = lw2[2 * u], l1 = lw2[2 * u + 1];
+          a0 = fmaT((T)v[4 * u + 0], (T)l0.x, a0);
+          a1 = fmaT((T)v[4 * u + 1], (T)l0.y, a1);
+          a2 = fmaT((T)v[4 * u + 2], (T)l1.x, a2);
+          a3 = fmaT((T)v[4 * u + 3], (T)l1.y, a3);
+        }
+      } else {
+        const float4 *lw4 = reinterpret_cast<const float4 *>(sh.lw + r);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float4 l = lw4[u];
+          a0 = fmaT((T)v[4 * u + 0], (T)l.x, a0);
+          a1 = fmaT((T)v[4 * u + 1], (T)l.y, a1);
+          a2 = fmaT((T)v[4 * u + 2], (T)l.z, a2);
+          a3 = fmaT((T)v[4 * u + 3], (T)l.w, a3);
+        }
       }
     }
     hc = (a0 + a1) + (a2 + a3);
@@ -997,6 +1011,7 @@ template <typename T>
 struct K2Shared {
   float lvn[K2];     // next Lanczos vector, unnormalised: the matvec input
   float lw[K2];      // w broadcast (also the index staging of the gather)
+  T lwT[sizeof(T) == 8 ? K2 : 4];  // w in T (FP64 coefficients only): one conversion per entry
   float pw[2][K2];   // matvec partials of the two halves
   float part[2][K2];  // Gram-Schmidt update partials of the two halves
   float lh[K2];       // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
@@ -1149,6 +1164,7 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
   if constexpr (WANT_PRE) ws = g_sum_dpp<64>(w * w);
   if (h == 0) {
     sh.lw[i] = w;
+    if constexpr (sizeof(T) == 8) sh.lwT[i] = (T)w;
     if (WANT_PRE && q == 0) sh.rp[wv] = ws;
   }
   __syncthreads();
@@ -1173,14 +1189,26 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
         float v[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) v[u] = col[u * K2S];
-        const float4 *lw4 = reinterpret_cast<const float4 *>(sh.lw + r0 + r);
+        if constexpr (sizeof(T) == 8) {
+          const double2 *lw2 = reinterpret_cast<const double2 *>(sh.lwT + r0 + r);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const float4 l = lw4[u];
-          a0 = fmaT((T)v[4 * u + 0], (T)l.x, a0);
-          a1 = fmaT((T)v[4 * u + 1], (T)l.y, a1);
-          a2 = fmaT((T)v[4 * u + 2], (T)l.z, a2);
-          a3 = fmaT((T)v[4 * u + 3], (T)l.w, a3);
+          for (int u = 0; u < 4; ++u) {
+            const double2 l0 = lw2[2 * u], l1 = lw2[2 * u + 1];
+            a0 = fmaT((T)v[4 * u + 0], (T)l0.x, a0);
+            a1 = fmaT((T)v[4 * u + 1], (T)l0.y, a1);
+            a2 = fmaT((T)v[4 * u + 2], (T)l1.x, a2);
+            a3 = fmaT((T)v[4 * u + 3], (T)l1.y, a3);
+          }
+        } else {
+          const float4 *lw4 = reinterpret_cast<const float4 *>(sh.lw + r0 + r);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const float4 l = lw4[u];
+            a0 = fmaT((T)v[4 * u + 0], (T)l.x, a0);
+            a1 = fmaT((T)v[4 * u + 1], (T)l.y, a1);
+            a2 = fmaT((T)v[4 * u + 2], (T)l.z, a2);
+            a3 = fmaT((T)v[4 * u + 3], (T)l.w, a3);
+          }
         }
       }
       sh.cp[g * C + c] = (a0 + a1) + (a2 + a3);
