@@ -1215,11 +1215,13 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
     }
   }
   // (2) register columns K2L + 2m + h (m < nm): products reduce-scattered per
-  // chunk; columns not yet formed hold zeros, so every chunk runs unguarded
+  // chunk, the chunks holding formed columns only (uniform branch per chunk;
+  // unformed columns inside a chunk hold zeros)
   const int nm = (max(0, j + 1 - K2L) + 1) >> 1;
   if (nm > 0) {
 #pragma unroll
     for (int c = 0; c < (K2R + NC - 1) / NC; ++c) {
+      if (NC * c >= nm) break;
       T x[NC];
 #pragma unroll
       for (int u = 0; u < NC; ++u) {
@@ -1281,9 +1283,13 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
   }
   if (nm > 0) {
 #pragma unroll
-    for (int m = 0; m < K2R; m += 2) {
-      u0 = __builtin_fmaf(vr[m], sh.lh[K2L + 2 * m + h], u0);
-      if (m + 1 < K2R) u1 = __builtin_fmaf(vr[m + 1], sh.lh[K2L + 2 * m + 2 + h], u1);
+    for (int m0 = 0; m0 < K2R; m0 += 8) {
+      if (m0 >= nm) break;
+#pragma unroll
+      for (int m = m0; m < m0 + 8 && m < K2R; m += 2) {
+        u0 = __builtin_fmaf(vr[m], sh.lh[K2L + 2 * m + h], u0);
+        if (m + 1 < K2R) u1 = __builtin_fmaf(vr[m + 1], sh.lh[K2L + 2 * m + 2 + h], u1);
+      }
     }
   }
   sh.part[h][i] = (u0 + u1) + (u2 + u3);
