@@ -202,6 +202,22 @@ pl_status pl_server_destroy(pl_server *server);
 pl_status pl_solve_alleig(const float *G, int64_t n, int64_t ld, const int32_t *idx, int k,
                           int64_t nsub, double *eig, double *det, int32_t *err, void *stream);
 
+/*
+ * RIP statistics of a solved batch (the outputs of pl_solve*), per subset
+ * size, for the reference's consumer of the bounds (the SubSeq record,
+ * ParaLanczos0.h:82-90: per index-set size the lower and upper bound as mean
+ * and standard deviation). out is a device array of (kmax+1) x 8 doubles;
+ * row k (1 <= k <= kmax) holds {count, mean lambda_min, std lambda_min,
+ * mean lambda_max, std lambda_max, max delta, min lambda_min, max lambda_max}
+ * over the subsets of size k without a fatal status bit (NaN when count is 0);
+ * row 0 holds the number of subsets skipped. Sizes come from offsets (CSR) or
+ * are all k. The sums are FP64 atomics: the last bits of the means and
+ * deviations may vary from run to run; counts and extremes do not.
+ */
+pl_status pl_rip_stats(const float *lmin, const float *lmax, const float *delta,
+                       const uint32_t *status, const int64_t *offsets, int k, int64_t nsub,
+                       int kmax, double *out, void *stream);
+
 /* gather stage only: out[s*k*k + r*k + c] = G_S[r][c] (mode 0), or
    out[s] = sum of G_S (mode 1); uniform k */
 pl_status pl_gather(const float *G, int64_t n, int64_t ld, const int32_t *idx, int k,

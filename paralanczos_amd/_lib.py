@@ -43,6 +43,7 @@ SIGNATURES = {
     "pl_server_poll": (ST, [P, I64, P]),
     "pl_server_query": (ST, [P, P, INT]),
     "pl_server_destroy": (ST, [P]),
+    "pl_rip_stats": (ST, [P, P, P, P, P, INT, I64, INT, P, P]),
     "pl_gather": (ST, [P, I64, I64, P, INT, I64, P, INT, INT, INT, P]),
     "pl_gen_subsets": (ST, [U64, I64, I64, INT, INT, P, P]),
     "pl_gen_subsets_csr": (ST, [U64, I64, I64, P, INT, P, P]),

@@ -635,6 +635,25 @@ pl_status pl_solve_alleig(const float *G, int64_t n, int64_t ld, const int32_t *
   return PL_OK;
 }
 
+pl_status pl_rip_stats(const float *lmin, const float *lmax, const float *delta,
+                       const uint32_t *status, const int64_t *offsets, int k, int64_t nsub,
+                       int kmax, double *out, void *stream) {
+  if (!lmin || !lmax || !delta || !status || !out)
+    return fail(PL_ERR_ARG, "pointers must be non-null");
+  if (kmax < 1 || kmax > kMaxK) return fail(PL_ERR_ARG, "kmax = %d outside [1, %d]", kmax, kMaxK);
+  if (!offsets && (k < 1 || k > kmax)) return fail(PL_ERR_ARG, "k = %d outside [1, kmax]", k);
+  if (nsub < 0) return fail(PL_ERR_ARG, "nsub < 0");
+  hipStream_t s = (hipStream_t)stream;
+  unsigned *ext = nullptr;
+  hipError_t e = hipMallocAsync((void **)&ext, sizeof(unsigned) * 3 * (kmax + 1), s);
+  if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(rip scratch)");
+  e = pl::launch_rip_stats(lmin, lmax, delta, status, offsets, offsets ? 0 : k, nsub, kmax, out,
+                           ext, s);
+  (void)hipFreeAsync(ext, s);
+  if (e != hipSuccess) return hip_fail(e, "rip statistics launch");
+  return PL_OK;
+}
+
 pl_status pl_gather(const float *G, int64_t n, int64_t ld, const int32_t *idx, int k,
                     int64_t nsub, float *out, int mode, int lanes, int pad, void *stream) {
   pl_status st = check_solve_args(G, n, ld, idx, k, nsub);

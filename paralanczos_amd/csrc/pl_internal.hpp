@@ -105,6 +105,12 @@ hipError_t launch_binning(const int64_t *offsets, int kuni, int64_t nsub, const 
 // mode 0: lambda_min/lambda_max/delta; mode 1: every eigenvalue + det (uniform k)
 hipError_t launch_bisect(int mode, const BisectArgs &a, hipStream_t s);
 
+// per-size RIP statistics of a solved batch into out[(kmax+1) x 8] (FP64);
+// ext is device scratch of (kmax+1) x 3 unsigned
+hipError_t launch_rip_stats(const float *lmin, const float *lmax, const float *delta,
+                            const uint32_t *status, const int64_t *offsets, int kuni,
+                            int64_t nsub, int kmax, double *out, unsigned *ext, hipStream_t s);
+
 hipError_t launch_gen_subsets(uint64_t seed, int64_t s0, int64_t nsub, const int64_t *offsets,
                               int kuni, int n, int32_t *idx, hipStream_t s);
 hipError_t launch_gen_ks(uint64_t seed, int64_t s0, int64_t nsub, const int32_t *klist,

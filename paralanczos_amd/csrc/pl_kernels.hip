@@ -1931,6 +1931,113 @@ hipError_t launch_bisect(int mode, const BisectArgs &a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// RIP statistics of a solved batch, per subset size (the per-index-set
+// lower/upper bound means and deviations the reference's SubSeq record holds,
+// ParaLanczos0.h:82-90, and the largest deviation delta). Block-local FP64
+// sums in LDS, one global atomic per field, size and block, then a finalize
+// pass. Row layout (RIP_FIELDS doubles per size): count, sum/mean lmin,
+// sumsq/std lmin, sum/mean lmax, sumsq/std lmax, max delta, min lmin, max lmax.
+// Row 0 counts the subsets skipped (fatal status or size outside [1, kmax]).
+// ---------------------------------------------------------------------------
+constexpr int RIP_FIELDS = 8;
+constexpr int RIP_THREADS = 256;
+
+// order-preserving map of a float onto unsigned (for atomic min/max)
+__device__ __forceinline__ unsigned f2ord(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(unsigned o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+__global__ void rip_clear_kernel(double *out, unsigned *ext, int kmax) {
+  for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < (kmax + 1) * RIP_FIELDS;
+       i += blockDim.x * gridDim.x)
+    out[i] = 0.0;
+  for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < (kmax + 1) * 3;
+       i += blockDim.x * gridDim.x)
+    ext[i] = (i % 3 == 1) ? f2ord(INFINITY) : f2ord(-INFINITY);  // max delta, min lmin, max lmax
+}
+
+__global__ __launch_bounds__(RIP_THREADS) void rip_accumulate_kernel(
+    const float *lmin, const float *lmax, const float *delta, const uint32_t *status,
+    const int64_t *offsets, int kuni, int64_t nsub, int kmax, double *out, unsigned *ext) {
+  extern __shared__ double acc[];  // (kmax+1) x 5 sums, then (kmax+1) x 3 ordered extremes
+  unsigned *lext = reinterpret_cast<unsigned *>(acc + (kmax + 1) * 5);
+  for (int i = threadIdx.x; i < (kmax + 1) * 5; i += RIP_THREADS) acc[i] = 0.0;
+  for (int i = threadIdx.x; i < (kmax + 1) * 3; i += RIP_THREADS)
+    lext[i] = (i % 3 == 1) ? f2ord(INFINITY) : f2ord(-INFINITY);
+  __syncthreads();
+  constexpr uint32_t FATAL = ST_BADK | ST_BADINDEX | ST_NONFINITE;
+  for (int64_t s = (int64_t)blockIdx.x * RIP_THREADS + threadIdx.x; s < nsub;
+       s += (int64_t)gridDim.x * RIP_THREADS) {
+    const int64_t k = offsets ? offsets[s + 1] - offsets[s] : (int64_t)kuni;
+    if (k < 1 || k > kmax || (status[s] & FATAL)) {
+      atomicAdd(&acc[0], 1.0);
+      continue;
+    }
+    const double lo = lmin[s], hi = lmax[s];
+    double *a = acc + k * 5;
+    atomicAdd(&a[0], 1.0);
+    atomicAdd(&a[1], lo);
+    atomicAdd(&a[2], lo * lo);
+    atomicAdd(&a[3], hi);
+    atomicAdd(&a[4], hi * hi);
+    atomicMax(&lext[k * 3 + 0], f2ord(delta[s]));
+    atomicMin(&lext[k * 3 + 1], f2ord(lmin[s]));
+    atomicMax(&lext[k * 3 + 2], f2ord(lmax[s]));
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < (kmax + 1) * 5; i += RIP_THREADS) {
+    if (acc[i] != 0.0) atomicAdd(&out[(i / 5) * RIP_FIELDS + (i % 5)], acc[i]);
+  }
+  for (int i = threadIdx.x; i < (kmax + 1) * 3; i += RIP_THREADS) {
+    if ((i % 3) == 1) atomicMin(&ext[i], lext[i]);
+    else atomicMax(&ext[i], lext[i]);
+  }
+}
+
+__global__ void rip_finalize_kernel(double *out, const unsigned *ext, int kmax) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > kmax) return;
+  double *r = out + (size_t)k * RIP_FIELDS;
+  if (k == 0) {
+    for (int f = 1; f < RIP_FIELDS; ++f) r[f] = 0.0;
+    return;
+  }
+  const double n = r[0];
+  if (n == 0.0) {
+    for (int f = 1; f < RIP_FIELDS; ++f) r[f] = NAN;
+    return;
+  }
+  const double m1 = r[1] / n, m2 = r[3] / n;
+  const double v1 = fmax(r[2] / n - m1 * m1, 0.0), v2 = fmax(r[4] / n - m2 * m2, 0.0);
+  r[1] = m1;
+  r[2] = sqrt(v1);
+  r[3] = m2;
+  r[4] = sqrt(v2);
+  r[5] = ord2f(ext[k * 3 + 0]);
+  r[6] = ord2f(ext[k * 3 + 1]);
+  r[7] = ord2f(ext[k * 3 + 2]);
+}
+
+hipError_t launch_rip_stats(const float *lmin, const float *lmax, const float *delta,
+                            const uint32_t *status, const int64_t *offsets, int kuni,
+                            int64_t nsub, int kmax, double *out, unsigned *ext, hipStream_t s) {
+  hipLaunchKernelGGL(rip_clear_kernel, dim3(4), dim3(256), 0, s, out, ext, kmax);
+  if (nsub > 0) {
+    const size_t lds = (size_t)(kmax + 1) * (5 * sizeof(double) + 3 * sizeof(unsigned));
+    const int64_t blocks = std::min<int64_t>((nsub + RIP_THREADS - 1) / RIP_THREADS, 512);
+    hipLaunchKernelGGL(rip_accumulate_kernel, dim3((unsigned)blocks), dim3(RIP_THREADS), lds, s,
+                       lmin, lmax, delta, status, offsets, kuni, nsub, kmax, out, ext);
+  }
+  hipLaunchKernelGGL(rip_finalize_kernel, dim3((kmax + 1 + 255) / 256), dim3(256), 0, s, out, ext,
+                     kmax);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Generators
 // ---------------------------------------------------------------------------
 

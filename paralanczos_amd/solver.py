@@ -177,6 +177,24 @@ def solve_alleig(G, idx, k):
     return eig, det, err
 
 
+RIP_FIELDS = ("count", "lmin_mean", "lmin_std", "lmax_mean", "lmax_std", "delta_max",
+              "lmin_min", "lmax_max")
+
+
+def rip_stats(lmin, lmax, delta, status, k=None, offsets=None, kmax=None):
+    """Per-size statistics of a solved batch: a (kmax+1, 8) FP64 device tensor,
+    row k = (count, mean/std lambda_min, mean/std lambda_max, max delta,
+    min lambda_min, max lambda_max); row 0 counts the skipped subsets."""
+    L = load()
+    nsub = lmin.numel()
+    if kmax is None:
+        kmax = k if offsets is None else int((offsets[1:] - offsets[:-1]).max().item())
+    out = torch.empty((kmax + 1, 8), dtype=torch.float64, device=lmin.device)
+    check(L.pl_rip_stats(_p(lmin), _p(lmax), _p(delta), _p(status), _p(offsets),
+                         int(k or 0), nsub, int(kmax), _p(out), _stream()), "pl_rip_stats")
+    return out
+
+
 def gather(G, idx, k, mode="dump", lanes=0, pad=4):
     """Gather stage alone: the k x k submatrices (mode 'dump') or their entry
     sums (mode 'checksum')."""

@@ -134,3 +134,35 @@ def test_c4_bernoulli_sample(gpu):
             assert rel(lo, w[:, 0]).max() <= RTOL and rel(hi, w[:, -1]).max() <= RTOL, (k, r)
     del G
     torch.cuda.empty_cache()
+
+
+def test_rip_stats_per_size(gpu):
+    """Per-size RIP statistics of a solved mixed batch equal numpy's over the
+    same outputs; a subset with an out-of-range index is skipped (row 0)."""
+    n = 2048
+    G = solver.gram(solver.gen_phi("gauss", 0xA1, 512, n))
+    ks = solver.gen_ks(0xA2, 700, [16, 32, 64, 128, 256])
+    off = solver.offsets_from_ks(ks)
+    idx = solver.gen_subsets_csr(0xA3, off, n)
+    idx[int(off[5].item())] = n + 3  # subset 5 gets BADINDEX
+    lo, hi, de, st = solver.solve(G, idx, k=256, offsets=off)
+    out = solver.rip_stats(lo, hi, de, st, offsets=off, kmax=256).cpu().numpy()
+    lo, hi, de, st, kk = [t.cpu().numpy() for t in (lo, hi, de, st, ks)]
+    assert st[5] & 8 and out[0, 0] == 1
+    ok = (st & (2 | 8 | 16)) == 0
+    for k in (16, 32, 64, 128, 256):
+        m = ok & (kk == k)
+        r = out[k]
+        assert r[0] == m.sum()
+        np.testing.assert_allclose(r[[1, 2, 3, 4]], [lo[m].astype(np.float64).mean(),
+                                                    lo[m].astype(np.float64).std(),
+                                                    hi[m].astype(np.float64).mean(),
+                                                    hi[m].astype(np.float64).std()], rtol=1e-9)
+        assert r[5] == de[m].max() and r[6] == lo[m].min() and r[7] == hi[m].max()
+    assert np.isnan(out[17, 1]) and out[17, 0] == 0
+    # a second pass over the same outputs: counts and extremes identical, sums
+    # to the last bits of FP64 atomics
+    u = solver.rip_stats(*[torch.from_numpy(a).cuda() for a in (lo, hi, de, st)], offsets=off,
+                         kmax=256).cpu().numpy()
+    assert np.array_equal(u[:, [0, 5, 6, 7]], out[:, [0, 5, 6, 7]], equal_nan=True)
+    np.testing.assert_allclose(u[1:, :], out[1:, :], rtol=1e-12, equal_nan=True)
