@@ -21,6 +21,8 @@ Reported beside it, in the same JSON line:
     sample of the served outputs against FP64 LAPACK dsyevd;
   * roofline of the dominant kernel (the k = 256 gather + Lanczos kernel,
     FP64 Gram-Schmidt coefficients), timed with HIP events on its stream;
+  * the gather stage alone over the same subsets: algorithmic GB/s and the
+    HBM fetch rate the profiler's counters give for it (profiles/);
   * cpu_baseline: the FP64 oracle on the box's host cores over a bounded
     sample of the same subsets.
 
@@ -109,6 +111,8 @@ def main():
                     help="bound on the CPU baseline's work")
     ap.add_argument("--profile-dominant", action="store_true",
                     help="only run the dominant kernel's launches (for rocprof counter runs)")
+    ap.add_argument("--profile-gather", action="store_true",
+                    help="only run the gather stage over the dominant subsets (rocprof counters)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy this many MiB device to device and exit (calibrates the "
                          "profiler's byte counters against a transfer of known size)")
@@ -154,6 +158,23 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"profile_dominant": True, "subsets": n256}), flush=True)
         return 0
+    # stage (a) alone over the same subsets: the gather-and-checksum launch
+    gsum = solver.gather(G, idx256, DOMINANT_K, "checksum", 256, 4)
+    torch.cuda.synchronize()
+    if args.profile_gather:
+        for _ in range(3):
+            solver.gather(G, idx256, DOMINANT_K, "checksum", 256, 4)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_gather": True, "subsets": n256}), flush=True)
+        return 0
+    g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g0.record(stream)
+    for _ in range(3):
+        solver.gather(G, idx256, DOMINANT_K, "checksum", 256, 4)
+    g1.record(stream)
+    torch.cuda.synchronize()
+    gather_ms = g0.elapsed_time(g1) / 3
+    del gsum
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(3):
@@ -237,6 +258,18 @@ def main():
     if os.path.exists(prof):
         with open(prof) as f:
             traffic = json.load(f).get("traffic_bytes_per_launch")
+    # gather stage: algorithmic bytes (the k x k FP32 G_S + indices + one
+    # output), and the HBM bytes the profiler counts for the same launch
+    gB = n256 * (4 * DOMINANT_K * DOMINANT_K + 4 * DOMINANT_K + 4)
+    gather = {"ms": gather_ms, "subsets": n256,
+              "algorithmic_GBs": gB / (gather_ms / 1e3) / 1e9, "peak_GBs": HBM_PEAK_GBS}
+    gprof = os.path.join(ROOT, "profiles", "gather_k256_traffic.json")
+    if os.path.exists(gprof):
+        with open(gprof) as f:
+            fb = json.load(f).get("fetch_bytes_per_launch")
+        if fb:
+            gather["hbm_fetch_GBs"] = fb / (gather_ms / 1e3) / 1e9
+            gather["hbm_frac"] = gather["hbm_fetch_GBs"] / HBM_PEAK_GBS
 
     cpu = None
     if not args.no_cpu:
@@ -297,6 +330,7 @@ def main():
                      "launch_ms": dom_ms, "subsets_per_launch": n256},
         "roofline_hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS},
+        "gather": gather,
         "cpu_baseline": cpu,
         "setup_s": time.perf_counter() - t_setup - wall,
     }
