@@ -24,7 +24,11 @@ Reported beside it, in the same JSON line:
   * the gather stage alone over the same subsets: algorithmic GB/s and the
     HBM fetch rate the profiler's counters give for it (profiles/);
   * cpu_baseline: the FP64 oracle on the box's host cores over a bounded
-    sample of the same subsets.
+    sample of the same subsets;
+  * batching: the mixed k in {8..128} configuration (tuned binned persistent
+    form eager / from a graph / after autotuning, vs the plain form);
+  * precision: k = 128 and 256 with FP32 vs FP64 Gram-Schmidt coefficients,
+    throughput and error vs FP64 dsyevd.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--requests 64] [--no-cpu]
 """
@@ -97,6 +101,84 @@ def poisson_run(srv, reqs, rate, seed):
     return st["subsets"] / st["span_s"], st["p50_s"], st["p99_s"]
 
 
+def batching_leg(solver, torch, autotune=True):
+    """Mixed sizes k in {8, 16, ..., 128}: Phi 1024 x 8192 N(0,1/1024) with
+    unit columns (seed 0xC3), 2^20 subsets (sizes and index sets from streams
+    0xC3+1). Tuned binned persistent form (eager, and replayed from a graph),
+    the plain one-launch-per-k form, and the tuned form after autotuning."""
+    n, nsub = 8192, 1 << 20
+    G = solver.gram(solver.gen_phi("gauss", 0xC3, 1024, n))
+    ks = solver.gen_ks(0xC3 + 1, nsub, list(range(8, 129, 8)))
+    off = solver.offsets_from_ks(ks)
+    idx = solver.gen_subsets_csr(0xC3 + 1, off, n)
+    torch.cuda.synchronize()
+
+    def ms(fn, reps=2):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    out = solver.solve(G, idx, k=128, offsets=off)
+    res = {"workload": "Phi 1024x8192 N(0,1/1024) unit columns, G 8192^2 FP32, 2^20 subsets, "
+                       "k uniform in {8,16,...,128}", "subsets": nsub}
+    res["plain_ms"] = ms(lambda: solver.solve(G, idx, k=128, offsets=off, form="plain", out=out))
+    res["tuned_ms"] = ms(lambda: solver.solve(G, idx, k=128, offsets=off, out=out))
+    plan = solver.Plan(G, idx, k=128, offsets=off)
+    res["tuned_graph_ms"] = ms(plan.execute)
+    res["graph"] = plan.is_graph
+    plan.close()
+    if autotune:
+        t0 = time.perf_counter()
+        tune = solver.autotune(G, list(range(8, 129, 8)), sample=32768)
+        res["autotune_s"] = time.perf_counter() - t0
+        res["autotuned"] = {k: list(v[:2]) for k, v in tune.items()}
+        res["autotuned_ms"] = ms(lambda: solver.solve(G, idx, k=128, offsets=off, out=out))
+        solver.tuning_reset()
+    best = min(res["tuned_ms"], res["tuned_graph_ms"], res.get("autotuned_ms", 1e30))
+    res["solves_s_tuned_best"] = nsub / (best / 1e3)
+    res["solves_s_plain"] = nsub / (res["plain_ms"] / 1e3)
+    res["vs_plain"] = res["plain_ms"] / best
+    del G
+    torch.cuda.empty_cache()
+    return res
+
+
+def precision_leg(solver, torch, dev, sample=1024):
+    """k = 128 and 256 on Phi = Bernoulli +-1/sqrt(2048), 2048 x 16384 (seed
+    0xC4; G 16384^2 FP32, 1 GiB): FP32 vs FP64 Gram-Schmidt coefficients,
+    throughput over 2^16 (k = 128) / 2^14 (k = 256) subsets and the max and
+    median relative error of lambda_min/lambda_max against FP64 dsyevd of the
+    gathered submatrices on a seeded sample."""
+    n = 16384
+    G = solver.gram(solver.gen_phi("bernoulli", 0xC4, 2048, n))
+    res = {"workload": "Phi Bernoulli +-1/sqrt(2048) 2048x16384, G 16384^2 FP32 (1 GiB)"}
+    for k, nsub in ((128, 1 << 16), (256, 1 << 14)):
+        idx = solver.gen_subsets(0xC4 + 1, nsub, k, n)
+        A = solver.gather(G, idx[:sample * k], k, "dump", 256 if k > 128 else 128, 4)
+        w = np.linalg.eigvalsh(A.cpu().numpy().reshape(-1, k, k).astype(np.float64))
+        for form in ("f32", "f64"):
+            out = solver.solve(G, idx, k=k, reorth=form)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(2):
+                solver.solve(G, idx, k=k, reorth=form, out=out)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / 2
+            lo, hi = out[0][:sample].cpu().numpy(), out[1][:sample].cpu().numpy()
+            e = np.maximum(np.abs(lo - w[:, 0]) / np.abs(w[:, 0]),
+                           np.abs(hi - w[:, -1]) / np.abs(w[:, -1]))
+            res[f"k{k}_{form}"] = {"solves_s": nsub / dt, "ms": dt * 1e3, "subsets": nsub,
+                                   "max_rel_err": float(e.max()),
+                                   "median_rel_err": float(np.median(e)), "sample": sample}
+    del G
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +193,8 @@ def main():
                     help="bound on the CPU baseline's work")
     ap.add_argument("--profile-dominant", action="store_true",
                     help="only run the dominant kernel's launches (for rocprof counter runs)")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the batching and precision configurations")
     ap.add_argument("--profile-gather", action="store_true",
                     help="only run the gather stage over the dominant subsets (rocprof counters)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -271,6 +355,11 @@ def main():
             gather["hbm_fetch_GBs"] = fb / (gather_ms / 1e3) / 1e9
             gather["hbm_frac"] = gather["hbm_fetch_GBs"] / HBM_PEAK_GBS
 
+    legs = {}
+    if not args.no_legs:
+        legs["batching"] = batching_leg(solver, torch)
+        legs["precision"] = precision_leg(solver, torch, dev)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -331,6 +420,7 @@ def main():
         "roofline_hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS},
         "gather": gather,
+        **legs,
         "cpu_baseline": cpu,
         "setup_s": time.perf_counter() - t_setup - wall,
     }
