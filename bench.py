@@ -158,7 +158,7 @@ def precision_leg(solver, torch, dev, sample=1024):
     res = {"workload": "Phi Bernoulli +-1/sqrt(2048) 2048x16384, G 16384^2 FP32 (1 GiB)"}
     for k, nsub in ((128, 1 << 16), (256, 1 << 14)):
         idx = solver.gen_subsets(0xC4 + 1, nsub, k, n)
-        A = solver.gather(G, idx[:sample * k], k, "dump", 256 if k > 128 else 128, 4)
+        A = solver.gather(G, idx[:sample * k], k, "dump", 512 if k > 128 else 128, 4)
         w = np.linalg.eigvalsh(A.cpu().numpy().reshape(-1, k, k).astype(np.float64))
         for form in ("f32", "f64"):
             out = solver.solve(G, idx, k=k, reorth=form)
@@ -243,18 +243,18 @@ def main():
         print(json.dumps({"profile_dominant": True, "subsets": n256}), flush=True)
         return 0
     # stage (a) alone over the same subsets: the gather-and-checksum launch
-    gsum = solver.gather(G, idx256, DOMINANT_K, "checksum", 256, 4)
+    gsum = solver.gather(G, idx256, DOMINANT_K, "checksum", 512, 4)
     torch.cuda.synchronize()
     if args.profile_gather:
         for _ in range(3):
-            solver.gather(G, idx256, DOMINANT_K, "checksum", 256, 4)
+            solver.gather(G, idx256, DOMINANT_K, "checksum", 512, 4)
         torch.cuda.synchronize()
         print(json.dumps({"profile_gather": True, "subsets": n256}), flush=True)
         return 0
     g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     g0.record(stream)
     for _ in range(3):
-        solver.gather(G, idx256, DOMINANT_K, "checksum", 256, 4)
+        solver.gather(G, idx256, DOMINANT_K, "checksum", 512, 4)
     g1.record(stream)
     torch.cuda.synchronize()
     gather_ms = g0.elapsed_time(g1) / 3
@@ -324,7 +324,7 @@ def main():
         if len(sel) == 0:
             continue
         sub = np.concatenate([idx_h[off_h[s]:off_h[s + 1]] for s in sel]).astype(np.int32)
-        A = solver.gather(G, torch.from_numpy(sub).to(dev), k, "dump", 256 if k > 128 else
+        A = solver.gather(G, torch.from_numpy(sub).to(dev), k, "dump", 512 if k > 128 else
                           (128 if k > 64 else 64), 4).cpu().numpy().reshape(-1, k, k)
         w = np.linalg.eigvalsh(A.astype(np.float64))
         e = np.maximum(np.abs(lmin[sel] - w[:, 0]) / np.abs(w[:, 0]),
@@ -415,7 +415,7 @@ def main():
                      "nonzero_status": int(np.count_nonzero(status))},
         "roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "solve_k256_kernel<TRIDIAG, double>",
+                     "kernel": "solve_halfrow_kernel<256, TRIDIAG, double>",
                      "launch_ms": dom_ms, "subsets_per_launch": n256},
         "roofline_hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS},

@@ -45,8 +45,10 @@ typedef enum {
 #define PL_SUB_BADK 16u     /* subset size outside the supported range    */
 
 /* solve forms */
-#define PL_FORM_TUNED 0 /* lane group and LDS pad from the tuning table; FP64
-                           Gram-Schmidt coefficients from k = 128 on            */
+#define PL_FORM_TUNED 0 /* variant (lanes, LDS pad) from the tuning table: lane
+                           groups up to k = 64, half-row kernels of 256 / 512
+                           lanes beyond; FP64 Gram-Schmidt coefficients from
+                           k = 128 on                                           */
 #define PL_FORM_PLAIN 1 /* one 64-lane wavefront per subset (128 / 512 lanes
                            beyond k = 64 / 128), FP32 throughout, one launch per k */
 /* precision flags, OR-ed into the form: FP32 or FP64 Gram-Schmidt
@@ -107,8 +109,10 @@ pl_status pl_solve_stage(const float *G, int64_t n, int64_t ld, const int32_t *i
                          float *delta, uint32_t *status, int form, int stage, void *workspace,
                          size_t workspace_bytes, void *stream);
 
-/* same, with an explicit variant: lanes per subset (8/16/32/64, >= k) and LDS
-   row padding (0, 1 or 4 floats) - used by the autotuner */
+/* same, with an explicit variant and FP32 coefficients: lanes per subset and
+   LDS row padding (0, 1 or 4 floats) - used by the autotuner. Variants: 8, 16,
+   32 or 64 lanes (k <= lanes; pad 0/1/4), 128 lanes (k <= 128; pad 1/4),
+   256 and 512 lanes (half rows: k <= 128 and k <= 256; pad 4) */
 pl_status pl_solve_variant(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                            const int64_t *offsets, int k, int64_t nsub, float *lmin,
                            float *lmax, float *delta, uint32_t *status, int lanes, int pad,
@@ -136,10 +140,10 @@ pl_status pl_plan_destroy(pl_plan *plan);
 /*
  * Tuning table of PL_FORM_TUNED: the variant (lanes per subset, LDS row
  * padding) used for each k; mixed batches are binned by runs of equal
- * variants. pl_autotune times every variant with lanes >= k of the family
- * that first covers k (lane groups of 8..64 lanes up to k = 64, the 128-lane
- * kernel up to 128, the 512-lane kernel beyond; a family gives bit-identical
- * results whatever its lanes and padding) and pad 0/1/4 on
+ * variants. pl_autotune times every variant that takes k of the family of
+ * k's default variant (lane groups of 8..64 lanes up to k = 64, the 256-lane
+ * half-row kernel up to 128, the 512-lane one beyond; a family gives
+ * bit-identical results whatever its lanes and padding) and pad 0/1/4 on
  * `sample` seeded subsets of each size in klist (nk sizes, each <= n), keeps
  * the fastest, and gives every unlisted size the winner of the next listed
  * size above it when that winner is of the size's own family (else the

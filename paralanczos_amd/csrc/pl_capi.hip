@@ -38,7 +38,12 @@ pl_status hip_fail(hipError_t e, const char *what) {
 
 constexpr int kMaxK = pl::KMAX_ALL;
 
+// lanes per subset of the default variant for k: the smallest lane group of
+// 8..64 lanes (one row per lane) that takes k, beyond k = 64 the half-row
+// kernels (256 lanes up to k = 128, 512 lanes up to 256)
 int group_for(int k) {
+  if (k > 128) return 512;
+  if (k > 64) return 256;
   int g = 8;
   while (g < k) g <<= 1;
   return g;
@@ -46,8 +51,8 @@ int group_for(int k) {
 
 // ---------------------------------------------------------------------------
 // Tuning table: the solve variant (lanes per subset, LDS row padding) used for
-// each k by the tuned form. Defaults: the smallest lane group >= k, pad 4;
-// pl_autotune overwrites entries with measured winners.
+// each k by the tuned form. Defaults: group_for(k) lanes, pad 4; pl_autotune
+// overwrites entries with measured winners.
 // ---------------------------------------------------------------------------
 using pl::Variant;
 std::mutex g_tune_mu;
@@ -56,11 +61,14 @@ bool g_tune_ready = false;
 
 Variant default_variant(int k) { return Variant{group_for(k), 4, false}; }
 
+// kernel family of a variant: results are bit-identical within a family
+int family(int lanes) { return lanes < 128 ? 0 : lanes; }
+
 constexpr int kFormMask = 0xff;
 constexpr int kPrecMask = PL_FORM_REORTH_F32 | PL_FORM_REORTH_F64;
 
 // FP64 Gram-Schmidt coefficients: forced by a flag, otherwise the tuned form
-// uses them from k = 128 on (lane groups of 128 and 256 carry them)
+// uses them from k = 128 on (variants of 128 lanes and more carry them)
 bool use_f64(int form, int k, int lanes) {
   if (lanes < 128) return false;
   if (form & PL_FORM_REORTH_F64) return true;
@@ -83,7 +91,7 @@ Variant tuned_variant(int k) {
 Variant plain_variant(int k) {
   // one 64-lane wavefront per subset; a 128-lane workgroup beyond k = 64 and
   // a 512-lane one beyond k = 128
-  return Variant{k <= 64 ? 64 : k <= 128 ? 128 : 256, 4, false};
+  return Variant{k <= 64 ? 64 : k <= 128 ? 128 : 512, 4, false};
 }
 
 // Bins of a mixed-size batch. Tuned: maximal runs of consecutive k sharing a
@@ -272,8 +280,9 @@ pl_status check_job(const SolveJob &j) {
   if (!j.lmin || !j.lmax || !j.delta || !j.status)
     return fail(PL_ERR_ARG, "output pointers must be non-null");
   if (j.form == -1) {
-    if (j.explicit_v.lanes < j.k)
-      return fail(PL_ERR_ARG, "lanes per subset %d < k = %d", j.explicit_v.lanes, j.k);
+    if (pl::variant_capacity(j.explicit_v.lanes) < j.k)
+      return fail(PL_ERR_ARG, "lanes per subset %d take k <= %d < k = %d", j.explicit_v.lanes,
+                  pl::variant_capacity(j.explicit_v.lanes), j.k);
     if (!pl::solve_variant_exists(j.explicit_v.lanes, j.explicit_v.pad))
       return fail(PL_ERR_ARG, "no variant with %d lanes and pad %d", j.explicit_v.lanes,
                   j.explicit_v.pad);
@@ -304,7 +313,8 @@ pl_status set_error(pl_status st, const char *msg) { return fail(st, "%s", msg);
 pl_status set_hip_error(hipError_t e, const char *what) { return hip_fail(e, what); }
 int lds_bytes_of(int form, int k) {
   const Variant v = job_variant(form, k);
-  if (v.lanes == 256) return 154 * 1024;
+  if (v.lanes == 512) return 154 * 1024;
+  if (v.lanes == 256) return 74 * 1024;
   if (v.lanes == 128) return 128 * (128 + v.pad) * 4;
   return v.lanes * (v.lanes + v.pad) * 4;
 }
@@ -481,7 +491,7 @@ pl_status pl_tuning_get(int k, int *lanes, int *pad) {
 
 pl_status pl_tuning_set(int k, int lanes, int pad) {
   if (k < 1 || k > kMaxK) return fail(PL_ERR_ARG, "k = %d outside [1, %d]", k, kMaxK);
-  if (lanes < k || !pl::solve_variant_exists(lanes, pad))
+  if (!pl::solve_variant_exists(lanes, pad) || pl::variant_capacity(lanes) < k)
     return fail(PL_ERR_ARG, "no variant with %d lanes and pad %d for k = %d", lanes, pad, k);
   std::lock_guard<std::mutex> lock(g_tune_mu);
   tune_init_locked();
@@ -518,7 +528,7 @@ pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klis
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   pl_status st = e == hipSuccess ? PL_OK : hip_fail(e, "autotune allocation");
-  static const int kLanes[] = {8, 16, 32, 64, 128, 256};
+  static const int kLanes[] = {8, 16, 32, 64, 128, 256, 512};
   static const int kPads[] = {0, 1, 4};
   for (int i = 0; i < nk && st == PL_OK; ++i) {
     const int k = klist[i];
@@ -531,10 +541,11 @@ pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klis
     float best_t = 0.f;
     for (int lanes : kLanes) {
       // the lane groups of 8..64 lanes compute bit-identical results, as do
-      // the 128-lane paddings; the 128- and 512-lane kernels (each with its
-      // own summation order) are offered only where the smaller family cannot
-      // take k, so tuning never changes a result
-      if (lanes < k || (lanes == 128 && k <= 64) || (lanes == 256 && k <= 128)) continue;
+      // the 128-lane paddings; the other kernels each have their own
+      // summation order, so only variants of the family that k's default
+      // belongs to are offered and tuning never changes a result
+      if (pl::variant_capacity(lanes) < k || family(lanes) != family(default_variant(k).lanes))
+        continue;
       for (int pad : kPads) {
         if (!pl::solve_variant_exists(lanes, pad)) continue;
         SolveJob j{G,   n, ld, idx, nullptr, k, sample, out, out + sample, out + 2 * sample,
@@ -575,7 +586,6 @@ pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klis
     std::vector<int> ks(klist, klist + nk);
     std::sort(ks.begin(), ks.end());
     ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
-    auto family = [](int lanes) { return lanes < 128 ? 0 : lanes == 128 ? 1 : 2; };
     std::lock_guard<std::mutex> lock(g_tune_mu);
     size_t q = 0;
     for (int k = 1; k <= ks.back(); ++k) {
@@ -661,7 +671,7 @@ pl_status pl_gather(const float *G, int64_t n, int64_t ld, const int32_t *idx, i
   if (!out) return fail(PL_ERR_ARG, "out must be non-null");
   if (mode != 0 && mode != 1) return fail(PL_ERR_ARG, "gather mode must be 0 or 1");
   if (lanes == 0) lanes = group_for(k);
-  if (lanes < k || !pl::solve_variant_exists(lanes, pad))
+  if (!pl::solve_variant_exists(lanes, pad) || pl::variant_capacity(lanes) < k)
     return fail(PL_ERR_ARG, "no variant with %d lanes and pad %d for k = %d", lanes, pad, k);
   pl::SolveArgs a{};
   a.G = G;

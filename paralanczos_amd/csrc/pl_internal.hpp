@@ -89,14 +89,17 @@ inline WorkspaceParts split_workspace(void *p, int64_t nsub, int kstride) {
   return w;
 }
 
-// lanes per subset (8, 16, 32, 64, 128, 256) x LDS row padding (0, 1, 4) x mode.
+// lanes per subset (8, 16, 32, 64, 128; 256 and 512 for the half-row kernels
+// of capacity 128 and 256) x LDS row padding (0, 1, 4) x mode.
 // With a.bins == nullptr the grid covers a.nslots slots; otherwise the launch
 // is persistent (grid = resident workgroups, capped by `max_slots`) and drains
 // bin a.bin of the table.
-// f64 selects FP64 Gram-Schmidt coefficients (lane groups of 128 and 256).
+// f64 selects FP64 Gram-Schmidt coefficients (128 lanes and more).
 hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStream_t s,
                         int64_t max_slots = 0, bool f64 = false);
 bool solve_variant_exists(int group, int pad);
+// largest k a variant with this many lanes per subset takes
+int variant_capacity(int group);
 
 // bin the nsub subsets (sizes from offsets, or kuni) by lut into table/order:
 // clear, count, scan, scatter - four stream-ordered launches, no host sync

@@ -978,25 +978,35 @@ __global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Large subsets (128 < k <= 256): one 512-lane workgroup (eight wavefronts,
-// one subset per CU). Lane t = (row i = t % 256, half h = t / 256) holds
-// G_S[i][128h .. 128h+127] in VGPRs. FP32 G_S and an FP32 basis do not both
-// fit one CU's LDS, so the Lanczos basis V is split: columns 0..K2L-1 live in
-// LDS (row-major, stride K2S), columns K2L..255 in VGPRs, column K2L + 2m + h
-// of row i as vr[m] on lane (i, h). Gram-Schmidt coefficients of the LDS
-// columns are column dot products (lane t forms column t % 256's over the
-// rows of half t / 256); those of the register columns come from a wavefront
-// reduce-scatter of the lanes' products (wave_rs) and a sum over the four
-// wavefronts of each half in LDS. The coefficients are accumulated in T
-// (float, or double: FP64 reorthogonalisation coefficients) and applied in
-// an FP32 update.
+// Half-row kernels, subset capacity KC = 128 or 256: one workgroup of 2 KC
+// lanes per subset. Lane t = (row i = t % KC, half h = t / KC) holds
+// G_S[i][KC/2 h .. KC/2 h + KC/2 - 1] in VGPRs. The Lanczos basis V keeps
+// columns 0..L-1 in LDS (row-major, stride S). At KC = 256 an FP32 G_S and an
+// FP32 basis do not both fit one CU's LDS, so columns L..255 live in VGPRs,
+// column L + 2m + h of row i as vr[m] on lane (i, h); at KC = 128 the whole
+// basis is in LDS and two subsets share a CU. Gram-Schmidt coefficients of
+// the LDS columns are column dot products (the lanes split into columns x row
+// groups); those of the register columns come from a wavefront reduce-scatter
+// of the lanes' products (wave_rs) and a sum over the wavefronts of each half
+// in LDS. The coefficients are accumulated in T (float, or double: FP64
+// reorthogonalisation coefficients) and applied in an FP32 update.
 // ---------------------------------------------------------------------------
-constexpr int K2 = 256;              // subset capacity
-constexpr int K2T = 512;             // threads per workgroup
-constexpr int K2L = 140;             // basis columns in LDS
-constexpr int K2S = 140;             // LDS row stride: 16-byte rows, stride/4 odd
-constexpr int K2R = (K2 - K2L) / 2;  // register basis columns per lane
-constexpr int K2H = 72;              // LDS columns updated by half 0 (rest: half 1)
+template <int KC>
+struct HR {
+  static_assert(KC == 128 || KC == 256, "half-row capacity");
+  static constexpr int NT = 2 * KC;             // threads per workgroup
+  static constexpr int HW = KC / 2;             // G_S columns per lane
+  static constexpr int WPH = KC / 64;           // wavefronts per half
+  static constexpr int NW = 2 * WPH;            // wavefronts
+  static constexpr int LOGK = KC == 256 ? 8 : 7;
+  static constexpr int L = KC == 256 ? 140 : 128;  // basis columns in LDS
+  static constexpr int S = KC == 256 ? 140 : 132;  // LDS row stride: 16-byte rows, stride/4 odd
+  static constexpr int R = (KC - L) / 2;        // register basis columns per lane
+  static constexpr int RA = R > 0 ? R : 1;      // their array extent
+  static constexpr int RP = R > 0 ? 64 : 1;     // register partial slots per (half, wavefront)
+  static_assert(R <= 64, "register partial slots");
+  static_assert(S % 4 == 0 && (S / 4) % 2 == 1 && L % 4 == 0 && L <= S, "row layout");
+};
 constexpr int DPP_ROW_ROR8 = 0x128;  // lane i <- lane (i + 8) mod 16 within a row
 
 // reduce-scatter chunk: 16 FP32 or 8 FP64 values (register budget)
@@ -1004,29 +1014,29 @@ template <typename T>
 constexpr int k2_chunk() {
   return sizeof(T) == 8 ? 8 : 16;
 }
-constexpr int K2RP = 64;  // register-column partial slots per (half, wavefront)
-static_assert(K2R <= K2RP, "register partial slots");
 
-template <typename T>
-struct K2Shared {
-  float lvn[K2];     // next Lanczos vector, unnormalised: the matvec input
-  float lw[K2];      // w broadcast (also the index staging of the gather)
-  T lwT[sizeof(T) == 8 ? K2 : 4];  // w in T (FP64 coefficients only): one conversion per entry
-  float pw[2][K2];   // matvec partials of the two halves
-  float part[2][K2];  // Gram-Schmidt update partials of the two halves
-  float lh[K2];       // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
-  T cp[2 * K2];       // LDS-column dot partials: row group g, column c at g*C + c
-  T rpart[2][4][K2RP];  // register-column partials per half and wavefront
-  float ra[16];      // alpha partials (p, e) of the eight wavefronts
-  float rb[8];       // beta^2 partials (p, e) of the four wavefronts of half 0
-  float rp[4];       // ||w||^2 partials before a Gram-Schmidt pass
+template <int KC, typename T>
+struct HRShared {
+  using C = HR<KC>;
+  float lvn[KC];     // next Lanczos vector, unnormalised: the matvec input
+  float lw[KC];      // w broadcast (also the index staging of the gather)
+  T lwT[sizeof(T) == 8 ? KC : 4];  // w in T (FP64 coefficients only): one conversion per entry
+  float pw[2][KC];   // matvec partials of the two halves
+  float part[2][KC];  // Gram-Schmidt update partials of the two halves
+  float lh[KC];       // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
+  T cp[C::NT];        // LDS-column dot partials: row group g, column c at g*C + c
+  T rpart[2][C::WPH][C::RP];  // register-column partials per half and wavefront
+  float ra[2 * C::NW];  // alpha partials (p, e) of all wavefronts
+  float rb[2 * C::WPH];  // beta^2 partials (p, e) of the wavefronts of half 0
+  float rp[C::WPH];     // ||w||^2 partials before a Gram-Schmidt pass
   float red[16];
   double redd[16];
   int ired[8];
 };
-static_assert(K2 * K2S * sizeof(float) + sizeof(K2Shared<double>) <= 163840, "LDS budget");
-static_assert(K2S % 4 == 0 && (K2S / 4) % 2 == 1 && K2H % 4 == 0 && (K2L - K2H) % 4 == 0,
-              "row layout");
+static_assert(256 * HR<256>::S * sizeof(float) + sizeof(HRShared<256, double>) <= 163840,
+              "LDS budget, capacity 256");
+static_assert(2 * (128 * HR<128>::S * sizeof(float) + sizeof(HRShared<128, double>)) <= 163840,
+              "LDS budget, capacity 128: two workgroups per CU");
 
 template <typename T>
 __device__ __forceinline__ T swap_add(T a, T b, bool wide32);
@@ -1101,24 +1111,29 @@ __device__ __forceinline__ T wave_rs(const T (&x)[N], int q) {
   return r;
 }
 
-template <bool MAX, typename T>
-__device__ __forceinline__ double k2_minmax(double x, int slot, K2Shared<T> &sh) {
+// min or max of x over the rows of half 0 (all lanes return it)
+template <bool MAX, int KC, typename T>
+__device__ __forceinline__ double hr_minmax(double x, int slot, HRShared<KC, T> &sh) {
+  constexpr int WPH = HR<KC>::WPH;
   x = MAX ? g_max<64>(x) : g_min<64>(x);
   const int t = threadIdx.x;
   __syncthreads();
-  if (t < K2 && (t & 63) == 0) sh.redd[4 * slot + (t >> 6)] = x;
+  if (t < KC && (t & 63) == 0) sh.redd[WPH * slot + (t >> 6)] = x;
   __syncthreads();
-  const double *r = sh.redd + 4 * slot;
-  return MAX ? fmax(fmax(r[0], r[1]), fmax(r[2], r[3]))
-             : fmin(fmin(r[0], r[1]), fmin(r[2], r[3]));
+  const double *r = sh.redd + WPH * slot;
+  double y = r[0];
+#pragma unroll
+  for (int u = 1; u < WPH; ++u) y = MAX ? fmax(y, r[u]) : fmin(y, r[u]);
+  return y;
 }
 
-// sum_c g[c] * L[c] over 128 entries, L a 16-byte aligned LDS vector read by
+// sum_c g[c] * L[c] over N entries, L a 16-byte aligned LDS vector read by
 // broadcast; four accumulators by index mod 4
-__device__ __forceinline__ float k2_rowdot(const float (&g)[128], const float *L) {
+template <int N>
+__device__ __forceinline__ float hr_rowdot(const float (&g)[N], const float *L) {
   float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
 #pragma unroll
-  for (int q = 0; q < 32; ++q) {
+  for (int q = 0; q < N / 4; ++q) {
     const float4 t = reinterpret_cast<const float4 *>(L)[q];
     a0 = __builtin_fmaf(g[4 * q + 0], t.x, a0);
     a1 = __builtin_fmaf(g[4 * q + 1], t.y, a1);
@@ -1130,12 +1145,12 @@ __device__ __forceinline__ float k2_rowdot(const float (&g)[128], const float *L
 
 // beta^2 = w.w (compensated, over the rows of half 0) and the next matvec
 // input lvn = w, published together behind one barrier
-template <typename T>
-__device__ __forceinline__ float k2_beta2(float w, K2Shared<T> &sh) {
-  const int t = threadIdx.x, i = t & (K2 - 1);
+template <int KC, typename T>
+__device__ __forceinline__ float hr_beta2(float w, HRShared<KC, T> &sh) {
+  const int t = threadIdx.x, i = t & (KC - 1);
   float p, e;
   g_dot2_pe<64>(w, w, p, e);
-  if (t < K2) {
+  if (t < KC) {
     sh.lvn[i] = w;
     if ((t & 63) == 0) {
       sh.rb[2 * (t >> 6)] = p;
@@ -1144,9 +1159,8 @@ __device__ __forceinline__ float k2_beta2(float w, K2Shared<T> &sh) {
   }
   __syncthreads();
   float p0 = sh.rb[0], e0 = sh.rb[1];
-  comp_step(p0, e0, sh.rb[2], sh.rb[3]);
-  comp_step(p0, e0, sh.rb[4], sh.rb[5]);
-  comp_step(p0, e0, sh.rb[6], sh.rb[7]);
+#pragma unroll
+  for (int u = 1; u < HR<KC>::WPH; ++u) comp_step(p0, e0, sh.rb[2 * u], sh.rb[2 * u + 1]);
   return p0 + e0;
 }
 
@@ -1154,12 +1168,15 @@ __device__ __forceinline__ float k2_beta2(float w, K2Shared<T> &sh) {
 // halves) against basis columns 0..j; returns the new entry. With WANT_PRE,
 // ||w||^2 before the pass is returned in pre (its partials ride along with
 // the broadcast of w).
-template <typename T, bool WANT_PRE>
-__device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
-                                        const float (&vr)[K2R], K2Shared<T> &sh, float &pre) {
+template <int KC, typename T, bool WANT_PRE>
+__device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
+                                        const float (&vr)[HR<KC>::RA], HRShared<KC, T> &sh,
+                                        float &pre) {
+  using C = HR<KC>;
   constexpr int NC = k2_chunk<T>();
   constexpr int SH = NC == 16 ? 2 : 3;
-  const int t = threadIdx.x, i = t & (K2 - 1), h = t >> 8, wv = (t >> 6) & 3, q = t & 63;
+  const int t = threadIdx.x, i = t & (KC - 1), h = t >> C::LOGK, wv = (t >> 6) & (C::WPH - 1),
+            q = t & 63;
   float ws = 0.f;
   if constexpr (WANT_PRE) ws = g_sum_dpp<64>(w * w);
   if (h == 0) {
@@ -1168,27 +1185,33 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
     if (WANT_PRE && q == 0) sh.rp[wv] = ws;
   }
   __syncthreads();
-  if constexpr (WANT_PRE) pre = ((sh.rp[0] + sh.rp[1]) + sh.rp[2]) + sh.rp[3];
-  // (1) LDS columns 0..ncl-1: the 512 lanes split into C columns x 512/C row
-  // groups of C/2 rows (C = 64, 128 or 256, the least that covers ncl), so
-  // every lane works and the dependent chains stay short; rows >= k are zero
-  const int ncl = min(j + 1, K2L);
+  if constexpr (WANT_PRE) {
+    float s = sh.rp[0];
+#pragma unroll
+    for (int u = 1; u < C::WPH; ++u) s += sh.rp[u];
+    pre = s;
+  }
+  // (1) LDS columns 0..ncl-1: the 2 KC lanes split into CC columns x 2 KC/CC
+  // row groups of CC/2 rows (CC = 64, 128 or 256, the least that covers
+  // ncl), so every lane works and the dependent chains stay short; rows >= k
+  // are zero
+  const int ncl = min(j + 1, C::L);
   const int lgC = ncl <= 64 ? 6 : ncl <= 128 ? 7 : 8;
-  const int C = 1 << lgC;
+  const int CC = 1 << lgC;
   {
-    const int c = t & (C - 1), g = t >> lgC;
+    const int c = t & (CC - 1), g = t >> lgC;
     if (c < ncl) {
       // 16 rows per iteration: their column entries are all loaded before the
       // FMAs consume them, so each iteration pays the LDS latency once
-      const int R = C >> 1;
-      const int r0 = g * R;
-      const int rn = (min(R, max(0, k - r0)) + 15) & ~15;
+      const int RR = CC >> 1;
+      const int r0 = g * RR;
+      const int rn = (min(RR, max(0, k - r0)) + 15) & ~15;
       T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
       for (int r = 0; r < rn; r += 16) {
-        const float *col = VL + (r0 + r) * K2S + c;
+        const float *col = VL + (r0 + r) * C::S + c;
         float v[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = col[u * K2S];
+        for (int u = 0; u < 16; ++u) v[u] = col[u * C::S];
         if constexpr (sizeof(T) == 8) {
           const double2 *lw2 = reinterpret_cast<const double2 *>(sh.lwT + r0 + r);
 #pragma unroll
@@ -1211,54 +1234,63 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
           }
         }
       }
-      sh.cp[g * C + c] = (a0 + a1) + (a2 + a3);
+      sh.cp[g * CC + c] = (a0 + a1) + (a2 + a3);
     }
   }
-  // (2) register columns K2L + 2m + h (m < nm): products reduce-scattered per
+  // (2) register columns L + 2m + h (m < nm): products reduce-scattered per
   // chunk, the chunks holding formed columns only (uniform branch per chunk;
   // unformed columns inside a chunk hold zeros)
-  const int nm = (max(0, j + 1 - K2L) + 1) >> 1;
-  if (nm > 0) {
+  int nm = 0;
+  if constexpr (C::R > 0) {
+    nm = (max(0, j + 1 - C::L) + 1) >> 1;
+    if (nm > 0) {
 #pragma unroll
-    for (int c = 0; c < (K2R + NC - 1) / NC; ++c) {
-      if (NC * c >= nm) break;
-      T x[NC];
+      for (int c = 0; c < (C::R + NC - 1) / NC; ++c) {
+        if (NC * c >= nm) break;
+        T x[NC];
 #pragma unroll
-      for (int u = 0; u < NC; ++u) {
-        const int m = NC * c + u;
-        x[u] = (m < K2R) ? (T)vr[m < K2R ? m : 0] * (T)w : (T)0;
+        for (int u = 0; u < NC; ++u) {
+          const int m = NC * c + u;
+          x[u] = (m < C::R) ? (T)vr[m < C::R ? m : 0] * (T)w : (T)0;
+        }
+        const T r = wave_rs<NC, T>(x, q);
+        if ((q & ((1 << SH) - 1)) == 0) sh.rpart[h][wv][NC * c + (q >> SH)] = r;
       }
-      const T r = wave_rs<NC, T>(x, q);
-      if ((q & ((1 << SH) - 1)) == 0) sh.rpart[h][wv][NC * c + (q >> SH)] = r;
     }
   }
   __syncthreads();
-  // (3) the coefficients, lane t < 256 for column t (zero beyond j); the row
+  // (3) the coefficients, lane t < KC for column t (zero beyond j); the row
   // groups' partials are summed in group order
   if (h == 0) {
     T hc = 0;
     if (i < ncl) {
-      const int ng = 512 >> lgC;
+      const int ng = C::NT >> lgC;
       hc = sh.cp[i];
-      for (int g = 1; g < ng; ++g) hc += sh.cp[g * C + i];
-    } else if (i >= K2L && i <= j) {
-      const int m = (i - K2L) >> 1, hh = (i - K2L) & 1;
-      hc = ((sh.rpart[hh][0][m] + sh.rpart[hh][1][m]) + sh.rpart[hh][2][m]) + sh.rpart[hh][3][m];
+      for (int g = 1; g < ng; ++g) hc += sh.cp[g * CC + i];
+    } else if constexpr (C::R > 0) {
+      if (i >= C::L && i <= j) {
+        const int m = (i - C::L) >> 1, hh = (i - C::L) & 1;
+        T s = sh.rpart[hh][0][m];
+#pragma unroll
+        for (int u = 1; u < C::WPH; ++u) s += sh.rpart[hh][u][m];
+        hc = s;
+      }
     }
     sh.lh[i] = (float)hc;
   }
   __syncthreads();
-  // (4) update partials (FP32): LDS columns [0, K2H) by half 0 and
-  // [K2H, K2L) by half 1, register columns by their owner half
+  // (4) update partials (FP32): half 0 takes LDS columns [0, cs) and half 1
+  // [cs, ncl), cs = ceil(ncl / 8) * 4; register columns go to their owner half
   float u0 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f;
   {
-    const int c0 = h == 0 ? 0 : K2H;
-    const int cn = h == 0 ? min(ncl, K2H) : max(0, ncl - K2H);
-    const float *row = VL + i * K2S + c0;
+    const int cs = ((ncl + 7) >> 3) << 2;
+    const int c0 = h == 0 ? 0 : cs;
+    const int cn = h == 0 ? min(ncl, cs) : max(0, ncl - cs);
+    const float *row = VL + i * C::S + c0;
     const float *hl = sh.lh + c0;
     // 8 columns per iteration, then at most one group of 4; reads stay in
-    // [c0, c0 + cn rounded up to 4), inside this half's share of the row
-    // (columns past j hold zeros and zero coefficients)
+    // [c0, c0 + cn rounded up to 4), inside this half's share of the row and
+    // below L (columns past j hold zeros and zero coefficients)
     const int cn4 = (cn + 3) & ~3;
     int c = 0;
     for (; c + 8 <= cn4; c += 8) {
@@ -1281,14 +1313,16 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
       u3 = __builtin_fmaf(ra.w, hl[c + 3], u3);
     }
   }
-  if (nm > 0) {
+  if constexpr (C::R > 0) {
+    if (nm > 0) {
 #pragma unroll
-    for (int m0 = 0; m0 < K2R; m0 += 8) {
-      if (m0 >= nm) break;
+      for (int m0 = 0; m0 < C::R; m0 += 8) {
+        if (m0 >= nm) break;
 #pragma unroll
-      for (int m = m0; m < m0 + 8 && m < K2R; m += 2) {
-        u0 = __builtin_fmaf(vr[m], sh.lh[K2L + 2 * m + h], u0);
-        if (m + 1 < K2R) u1 = __builtin_fmaf(vr[m + 1], sh.lh[K2L + 2 * m + 2 + h], u1);
+        for (int m = m0; m < m0 + 8 && m < C::R; m += 2) {
+          u0 = __builtin_fmaf(vr[m], sh.lh[C::L + 2 * m + h], u0);
+          if (m + 1 < C::R) u1 = __builtin_fmaf(vr[m + 1], sh.lh[C::L + 2 * m + 2 + h], u1);
+        }
       }
     }
   }
@@ -1297,10 +1331,11 @@ __device__ __forceinline__ float k2_cgs(float w, int j, int k, const float *VL,
   return w - (sh.part[0][i] + sh.part[1][i]);
 }
 
-template <int MODE, typename T>
-__device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Shared<T> &sh,
+template <int KC, int MODE, typename T>
+__device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShared<KC, T> &sh,
                                           int64_t s) {
-  const int t = threadIdx.x, i = t & (K2 - 1), h = t >> 8, q = t & 63;
+  using C = HR<KC>;
+  const int t = threadIdx.x, i = t & (KC - 1), h = t >> C::LOGK, q = t & 63;
   int k;
   int64_t off;
   if (a.offsets) {
@@ -1310,7 +1345,7 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
     k = a.kuni;
     off = s * (int64_t)k;
   }
-  const bool badk = k < 1 || k > K2 || k > a.kcap;
+  const bool badk = k < 1 || k > KC || k > a.kcap;
   uint32_t st = badk ? ST_BADK : 0u;
   int my = 0;
   if (!st && i < k) {
@@ -1322,12 +1357,12 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
   if (h == 0) lidx[i] = my;
   __syncthreads();
 
-  // ---- (a) gather: lane (i, h) loads G_S[i][128h + c] directly ----
-  float g[128];
+  // ---- (a) gather: lane (i, h) loads G_S[i][HW h + c] directly ----
+  float g[C::HW];
   bool fin = true;
 #pragma unroll
-  for (int c = 0; c < 128; ++c) {
-    const int col = 128 * h + c;
+  for (int c = 0; c < C::HW; ++c) {
+    const int col = C::HW * h + c;
     float x = 0.f;
     if (st == 0 && i < k && col < k) {
       const int ic = lidx[col];
@@ -1341,22 +1376,22 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
 
   if constexpr (MODE == MODE_DUMP) {
     if (i < k && st == 0) {
-      float *o = a.gs_out + s * (int64_t)k * k + (int64_t)i * k + 128 * h;
+      float *o = a.gs_out + s * (int64_t)k * k + (int64_t)i * k + C::HW * h;
 #pragma unroll
-      for (int c = 0; c < 128; ++c)
-        if (128 * h + c < k) o[c] = g[c];
+      for (int c = 0; c < C::HW; ++c)
+        if (C::HW * h + c < k) o[c] = g[c];
     }
     return;
   } else if constexpr (MODE == MODE_CHECKSUM) {
     float acc = 0.f;
 #pragma unroll
-    for (int c = 0; c < 128; ++c) acc += g[c];
+    for (int c = 0; c < C::HW; ++c) acc += g[c];
     acc = g_sum<64>(acc);
     if (q == 0) sh.red[t >> 6] = acc;
     __syncthreads();
     if (t == 0) {
       float sum = 0.f;
-      for (int w = 0; w < 8; ++w) sum += sh.red[w];
+      for (int w = 0; w < C::NW; ++w) sum += sh.red[w];
       a.gs_out[s] = st ? NAN : sum;
     }
     return;
@@ -1369,15 +1404,15 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
       // by 1/beta, with alpha's compensated partials exchanged alongside;
       // three-term update; Gram-Schmidt pass (||w||^2 rides along); beta^2
       // published with the next vector. Six barriers per step. ----
-      float vr[K2R];
+      float vr[C::RA];
 #pragma unroll
-      for (int m = 0; m < K2R; ++m) vr[m] = 0.f;
-      for (int e = t; e < K2 * K2S; e += K2T) VL[e] = 0.f;
+      for (int m = 0; m < C::RA; ++m) vr[m] = 0.f;
+      for (int e = t; e < KC * C::S; e += C::NT) VL[e] = 0.f;
       float scale = 1.0f / sqrtf((float)k);
       float vi = (i < k) ? scale : 0.f;
       __syncthreads();
       if (h == 0) {
-        VL[i * K2S] = vi;
+        VL[i * C::S] = vi;
         sh.lvn[i] = (i < k) ? 1.0f : 0.f;
       }
       __syncthreads();
@@ -1385,7 +1420,7 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
       for (int j = 0; j < k; ++j) {
         // matvec partial over this half's columns; alpha's partial Dot2 of v
         // with it (the two halves' partials sum to v . G lvn)
-        const float mv = k2_rowdot(g, sh.lvn + 128 * h);
+        const float mv = hr_rowdot<C::HW>(g, sh.lvn + C::HW * h);
         float p, e;
         g_dot2_pe<64>(vi, mv, p, e);
         sh.pw[h][i] = mv;
@@ -1397,18 +1432,18 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
         float w = (sh.pw[0][i] + sh.pw[1][i]) * scale;
         float p0 = sh.ra[0], e0 = sh.ra[1];
 #pragma unroll
-        for (int w2 = 1; w2 < 8; ++w2) comp_step(p0, e0, sh.ra[2 * w2], sh.ra[2 * w2 + 1]);
+        for (int w2 = 1; w2 < C::NW; ++w2) comp_step(p0, e0, sh.ra[2 * w2], sh.ra[2 * w2 + 1]);
         const float al = (p0 + e0) * scale;
         my_a = (i == j) ? al : my_a;
         amax = fmaxf(amax, fabsf(al));
         if (j == k - 1) break;
         w = __builtin_fmaf(-al, vi, __builtin_fmaf(-bj, vp, w));
         float pre = 0.f, unused;
-        w = k2_cgs<T, true>(w, j, k, VL, vr, sh, pre);
-        float b2 = k2_beta2(w, sh);
+        w = hr_cgs<KC, T, true>(w, j, k, VL, vr, sh, pre);
+        float b2 = hr_beta2(w, sh);
         if (b2 < 0.5f * pre) {  // DGKS: w was mostly in span(V); orthogonalise again
-          w = k2_cgs<T, false>(w, j, k, VL, vr, sh, unused);
-          b2 = k2_beta2(w, sh);
+          w = hr_cgs<KC, T, false>(w, j, k, VL, vr, sh, unused);
+          b2 = hr_beta2(w, sh);
         }
         const float b = sqrtf(b2);
         vp = vi;
@@ -1423,42 +1458,45 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
           bj = 0.f;
           float rs = 0.f;
           if (h == 0)
-            for (int l = 0; l < K2L; ++l) rs = __builtin_fmaf(VL[i * K2S + l], VL[i * K2S + l], rs);
+            for (int l = 0; l < C::L; ++l)
+              rs = __builtin_fmaf(VL[i * C::S + l], VL[i * C::S + l], rs);
+          if constexpr (C::R > 0) {
 #pragma unroll
-          for (int m = 0; m < K2R; ++m) rs = __builtin_fmaf(vr[m], vr[m], rs);
+            for (int m = 0; m < C::R; ++m) rs = __builtin_fmaf(vr[m], vr[m], rs);
+          }
           __syncthreads();
           sh.pw[h][i] = rs;
           __syncthreads();
           const float res = 1.f - (sh.pw[0][i] + sh.pw[1][i]);
           const int wb = restart_lane<64>(res, i, k);
           const float wvv = __shfl(i < k ? res : -1.f, wb & 63, 64);
-          if (t < K2 && q == 0) {
+          if (t < KC && q == 0) {
             sh.red[t >> 6] = wvv;
             sh.ired[t >> 6] = wb;
           }
           __syncthreads();
           int bi = sh.ired[0];
           float bv = sh.red[0];
-          for (int w2 = 1; w2 < 4; ++w2)
+          for (int w2 = 1; w2 < C::WPH; ++w2)
             if (sh.red[w2] > bv) {
               bv = sh.red[w2];
               bi = sh.ired[w2];
             }
           w = (i == bi) ? 1.f : 0.f;
-          w = k2_cgs<T, false>(w, j, k, VL, vr, sh, unused);
-          w = k2_cgs<T, false>(w, j, k, VL, vr, sh, unused);
-          scale = 1.0f / sqrtf(k2_beta2(w, sh));
+          w = hr_cgs<KC, T, false>(w, j, k, VL, vr, sh, unused);
+          w = hr_cgs<KC, T, false>(w, j, k, VL, vr, sh, unused);
+          scale = 1.0f / sqrtf(hr_beta2(w, sh));
         }
         vi = w * scale;
-        // basis column j + 1: LDS, or register slot (j + 1 - K2L) / 2 of one half
+        // basis column j + 1: LDS, or register slot (j + 1 - L) / 2 of one half
         const int c = j + 1;
-        if (c < K2L) {
-          if (h == 0) VL[i * K2S + c] = vi;
-        } else {
-          const int m = (c - K2L) >> 1;
-          const bool mine = ((c - K2L) & 1) == h;
+        if (c < C::L) {
+          if (h == 0) VL[i * C::S + c] = vi;
+        } else if constexpr (C::R > 0) {
+          const int m = (c - C::L) >> 1;
+          const bool mine = ((c - C::L) & 1) == h;
 #pragma unroll
-          for (int r = 0; r < K2R; ++r) vr[r] = (mine && r == m) ? vi : vr[r];
+          for (int r = 0; r < C::R; ++r) vr[r] = (mine && r == m) ? vi : vr[r];
         }
         my_b = (i == j + 1) ? bj : my_b;
       }
@@ -1468,7 +1506,7 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
         float2 *tri = a.tri + s * (int64_t)a.tstride;
         sh.pw[0][i] = my_b;  // both halves write the same value
         __syncthreads();
-        const double al = my_a, b = my_b, bn = (i + 1 < K2) ? (double)sh.pw[0][i + 1] : 0.0;
+        const double al = my_a, b = my_b, bn = (i + 1 < KC) ? (double)sh.pw[0][i + 1] : 0.0;
         double glo = INFINITY, ghi = -INFINITY, amin = INFINITY, amx = -INFINITY;
         if (i < k) {
           if (h == 0) tri[i] = make_float2(my_a, i > 0 ? my_b : 0.f);
@@ -1477,10 +1515,10 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
           ghi = al + r;
           amin = amx = al;
         }
-        glo = k2_minmax<false>(glo, 0, sh);
-        ghi = k2_minmax<true>(ghi, 1, sh);
-        amin = k2_minmax<false>(amin, 2, sh);
-        amx = k2_minmax<true>(amx, 3, sh);
+        glo = hr_minmax<false>(glo, 0, sh);
+        ghi = hr_minmax<true>(ghi, 1, sh);
+        amin = hr_minmax<false>(amin, 2, sh);
+        amx = hr_minmax<true>(amx, 3, sh);
         if (t == 0) a.bounds[s] = make_double4(glo, ghi, amin, amx);
       }
     }
@@ -1488,10 +1526,10 @@ __device__ __forceinline__ void k2_subset(const SolveArgs &a, float *VL, K2Share
   }
 }
 
-template <int MODE, typename T>
-__global__ __launch_bounds__(K2T) void solve_k256_kernel(SolveArgs a) {
-  __shared__ __attribute__((aligned(16))) float VL[K2 * K2S];
-  __shared__ __attribute__((aligned(16))) K2Shared<T> sh;
+template <int KC, int MODE, typename T>
+__global__ __launch_bounds__(2 * KC) void solve_halfrow_kernel(SolveArgs a) {
+  __shared__ __attribute__((aligned(16))) float VL[KC * HR<KC>::S];
+  __shared__ __attribute__((aligned(16))) HRShared<KC, T> sh;
   int64_t cnt = a.nslots;
   const int32_t *ord = a.order;
   if (a.bins) {
@@ -1499,7 +1537,7 @@ __global__ __launch_bounds__(K2T) void solve_k256_kernel(SolveArgs a) {
     ord = a.order + a.bins[BIN_START + a.bin];
   }
   for (int64_t slot = blockIdx.x; slot < cnt; slot += gridDim.x) {
-    k2_subset<MODE, T>(a, VL, sh, ord ? (int64_t)ord[slot] : slot);
+    hr_subset<KC, MODE, T>(a, VL, sh, ord ? (int64_t)ord[slot] : slot);
     __syncthreads();
   }
 }
@@ -1512,13 +1550,19 @@ struct VariantFns {
                                      solve_kernel<GROUP, PAD, 2>};
 };
 
+template <int KC>
+static SolveFn pick_halfrow(int mode, bool f64) {
+  static constexpr SolveFn f32[3] = {solve_halfrow_kernel<KC, 0, float>,
+                                     solve_halfrow_kernel<KC, 1, float>,
+                                     solve_halfrow_kernel<KC, 2, float>};
+  return (f64 && mode == MODE_TRIDIAG) ? solve_halfrow_kernel<KC, 0, double> : f32[mode];
+}
+
+// group = lanes per subset: 8..64 (lane groups), 128 (one row per lane),
+// 256 and 512 (half rows, capacities 128 and 256)
 static SolveFn pick(int group, int pad, int mode, bool f64) {
-  if (group == K2) {
-    if (pad != 4) return nullptr;
-    static constexpr SolveFn k32[3] = {solve_k256_kernel<0, float>, solve_k256_kernel<1, float>,
-                                       solve_k256_kernel<2, float>};
-    return (f64 && mode == MODE_TRIDIAG) ? solve_k256_kernel<0, double> : k32[mode];
-  }
+  if (group == 2 * 256) return pad == 4 ? pick_halfrow<256>(mode, f64) : nullptr;
+  if (group == 2 * 128) return pad == 4 ? pick_halfrow<128>(mode, f64) : nullptr;
   if (group == WIDE) {
     static constexpr SolveFn w1[3] = {solve_wide_kernel<1, 0>, solve_wide_kernel<1, 1>,
                                       solve_wide_kernel<1, 2>};
@@ -1544,6 +1588,8 @@ static SolveFn pick(int group, int pad, int mode, bool f64) {
 
 bool solve_variant_exists(int group, int pad) { return pick(group, pad, 0, false) != nullptr; }
 
+int variant_capacity(int group) { return group >= 256 ? group / 2 : group; }
+
 // workgroups of a variant that fit on the device at once (cached per variant)
 static int resident_blocks(SolveFn fn, int threads) {
   static std::mutex mu;
@@ -1568,7 +1614,7 @@ hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStr
   if (mode < 0 || mode > 2) return hipErrorInvalidValue;
   SolveFn fn = pick(group, pad, mode, f64);
   if (!fn) return hipErrorInvalidValue;
-  const int threads = group == K2 ? K2T : group == WIDE ? WIDE : BLOCK;
+  const int threads = group >= WIDE ? group : BLOCK;
   const int spb = group >= WIDE ? 1 : BLOCK / group;
   int64_t blocks;
   if (a.bins) {

@@ -42,18 +42,24 @@ def _np(ts):
 def test_tuned_plain_variant_bitexact(mixed):
     """With FP32 Gram-Schmidt coefficients at every k (the tuned form switches
     to FP64 ones at k = 128 by default) the tuned and plain forms agree bit for
-    bit: both use the lane-group family up to k = 64 and the 128-lane kernel
-    beyond. One 128-lane launch for every size agrees with them bit for bit
-    beyond k = 64 and to 1e-5 below (its own summation order)."""
+    bit up to k = 64, where both use the lane-group family. Beyond, the tuned
+    form uses the 256-lane half-row kernel and the plain form the 128-lane
+    kernel, each with its own summation order: they agree to 1e-5, and one
+    launch of either kernel over every size agrees bit for bit with the form
+    that uses it."""
     G, ks, off, idx = mixed
     tuned = _np(solver.solve(G, idx, k=128, offsets=off, form="tuned", reorth="f32"))
     plain = _np(solver.solve(G, idx, k=128, offsets=off, form="plain"))
-    single = _np(solver.solve(G, idx, k=128, offsets=off, lanes=128, pad=4))
+    s128 = _np(solver.solve(G, idx, k=128, offsets=off, lanes=128, pad=4))
+    s256 = _np(solver.solve(G, idx, k=128, offsets=off, lanes=256, pad=4))
     kk0 = ks.cpu().numpy()
-    for a, b, c in zip(tuned, plain, single):
-        assert np.array_equal(a, b, equal_nan=True)
-        assert np.array_equal(a[kk0 > 64], c[kk0 > 64], equal_nan=True)
-    assert rel(single[0], tuned[0]).max() <= RTOL and rel(single[1], tuned[1]).max() <= RTOL
+    lo, hi = kk0 <= 64, kk0 > 64
+    for a, b, c, d in zip(tuned, plain, s128, s256):
+        assert np.array_equal(a[lo], b[lo], equal_nan=True)
+        assert np.array_equal(a[hi], d[hi], equal_nan=True)
+        assert np.array_equal(b[hi], c[hi], equal_nan=True)
+    for x in (plain, s128, s256):
+        assert rel(x[0], tuned[0]).max() <= RTOL and rel(x[1], tuned[1]).max() <= RTOL
     assert np.all(tuned[3] == 0)
     # the default tuned form differs only in the k = 128 subsets, within 1e-5
     dflt = _np(solver.solve(G, idx, k=128, offsets=off))
@@ -138,12 +144,12 @@ def test_autotune(mixed):
             assert lanes >= k and pad in (0, 1, 4) and ms > 0
         # unlisted sizes take the winner of the next listed size above them
         # when it is of their own kernel family (lane groups up to 64 lanes,
-        # then the 128-lane kernel), else their default
-        fam = lambda lanes: 0 if lanes < 128 else 1
+        # then the 256-lane half-row kernel), else their default
+        fam = lambda lanes: 0 if lanes < 128 else lanes
         for k in range(1, 73):
             nxt = min(x for x in (8, 24, 40, 72) if x >= k)
             want = solver.tuning_get(nxt)
-            dflt = (max(8, 1 << (k - 1).bit_length()), 4)
+            dflt = (256 if k > 64 else max(8, 1 << (k - 1).bit_length()), 4)
             if fam(want[0]) != fam(dflt[0]):
                 want = dflt
             assert solver.tuning_get(k) == want, k

@@ -23,18 +23,19 @@ def _np(ts):
     return [t.cpu().numpy() for t in ts]
 
 
-def test_gather_dump_k256_bit_exact(gpu):
+def test_gather_dump_halfrow_bit_exact(gpu):
+    """The half-row kernels' gathers (256 lanes: k <= 128, 512 lanes: k <= 256)."""
     n = 1024
     G = solver.gram(solver.gen_phi("gauss", 3, 256, n))
     Gh = G.cpu().numpy()
-    for k in (129, 200, 256):
+    for lanes, k in ((256, 65), (256, 100), (256, 128), (512, 129), (512, 200), (512, 256)):
         idx = solver.gen_subsets(40 + k, 9, k, n)
-        out = solver.gather(G, idx, k, "dump", 256, 4).cpu().numpy().reshape(-1, k, k)
+        out = solver.gather(G, idx, k, "dump", lanes, 4).cpu().numpy().reshape(-1, k, k)
         ih = idx.cpu().numpy()
         for s in range(9):
             A, st = oracle.gather(Gh, ih[s * k:(s + 1) * k])
-            assert st == 0 and np.array_equal(out[s], A.astype(np.float32)), (k, s)
-        cs = solver.gather(G, idx, k, "checksum", 256, 4).cpu().numpy()
+            assert st == 0 and np.array_equal(out[s], A.astype(np.float32)), (lanes, k, s)
+        cs = solver.gather(G, idx, k, "checksum", lanes, 4).cpu().numpy()
         np.testing.assert_allclose(cs, out.reshape(9, -1).astype(np.float64).sum(1), rtol=1e-4)
 
 
@@ -47,7 +48,7 @@ def test_k256_vs_oracle(gpu, k):
     ih = idx.cpu().numpy()
     olo, ohi, _, ost = oracle.solve_batch(Gh, ih, k=k, nthreads=8)
     assert np.all(ost == 0)
-    runs = [solver.solve(G, idx, k=k, lanes=256, pad=4)]
+    runs = [solver.solve(G, idx, k=k, lanes=512, pad=4)]
     if k > 128:
         runs += [solver.solve(G, idx, k=k, reorth=r) for r in ("f32", "f64", None)]
     for r in runs:
@@ -98,7 +99,9 @@ def test_alleig_k256_full_rank(gpu):
 def test_mixed_up_to_256(gpu):
     """k from {16, 32, 64, 128, 256} in one CSR batch: the tuned form (FP64
     coefficients from 128) within 1e-5 of the oracle; with FP32 coefficients
-    forced it is bit-identical to the plain one-launch-per-k form."""
+    forced it is bit-identical to the plain one-launch-per-k form wherever
+    both use the same kernel (k <= 64: the lane groups; k = 256: the 512-lane
+    kernel) and within 1e-5 at k = 128 (256- vs 128-lane kernel)."""
     n = 4096
     G = solver.gram(solver.gen_phi("gauss", 0xC5, 1024, n))
     ks = solver.gen_ks(0xC5 + 1, 600, [16, 32, 64, 128, 256])
@@ -107,8 +110,10 @@ def test_mixed_up_to_256(gpu):
     tuned = _np(solver.solve(G, idx, k=256, offsets=off))
     plain = _np(solver.solve(G, idx, k=256, offsets=off, form="plain"))
     t32 = _np(solver.solve(G, idx, k=256, offsets=off, reorth="f32"))
+    same = ks.cpu().numpy() != 128
     for a, b in zip(t32, plain):
-        assert np.array_equal(a, b)
+        assert np.array_equal(a[same], b[same])
+    assert rel(t32[0], plain[0]).max() <= RTOL and rel(t32[1], plain[1]).max() <= RTOL
     o = off.cpu().numpy()
     olo, ohi, _, ost = oracle.solve_batch(G.cpu().numpy(), idx.cpu().numpy(), offsets=o,
                                           nthreads=8)
@@ -126,7 +131,7 @@ def test_c4_bernoulli_sample(gpu):
     G = solver.gram(solver.gen_phi("bernoulli", 0xC4, m, n))
     for k in (128, 256):
         idx = solver.gen_subsets(0xC4 + 2, 48, k, n)
-        A = solver.gather(G, idx, k, "dump", 256, 4).cpu().numpy().reshape(-1, k, k)
+        A = solver.gather(G, idx, k, "dump", 512, 4).cpu().numpy().reshape(-1, k, k)
         w = np.linalg.eigvalsh(A.astype(np.float64))
         for r in ("f32", "f64"):
             lo, hi, de, st = _np(solver.solve(G, idx, k=k, reorth=r))
