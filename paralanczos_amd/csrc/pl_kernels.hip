@@ -978,33 +978,37 @@ __global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Half-row kernels, subset capacity KC = 128 or 256: one workgroup of 2 KC
-// lanes per subset. Lane t = (row i = t % KC, half h = t / KC) holds
-// G_S[i][KC/2 h .. KC/2 h + KC/2 - 1] in VGPRs. The Lanczos basis V keeps
-// columns 0..L-1 in LDS (row-major, stride S). At KC = 256 an FP32 G_S and an
-// FP32 basis do not both fit one CU's LDS, so columns L..255 live in VGPRs,
-// column L + 2m + h of row i as vr[m] on lane (i, h); at KC = 128 the whole
-// basis is in LDS and two subsets share a CU. Gram-Schmidt coefficients of
-// the LDS columns are column dot products (the lanes split into columns x row
-// groups); those of the register columns come from a wavefront reduce-scatter
-// of the lanes' products (wave_rs) and a sum over the wavefronts of each half
-// in LDS. The coefficients are accumulated in T (float, or double: FP64
-// reorthogonalisation coefficients) and applied in an FP32 update.
+// Row-part kernels, subset capacity KC = 128 or 256, Q = 2 or 4 parts per
+// row: one workgroup of Q KC lanes per subset. Lane t = (row i = t % KC,
+// part h = t / KC) holds G_S[i][KC/Q h .. KC/Q (h + 1) - 1] in VGPRs. The
+// Lanczos basis V keeps columns 0..L-1 in LDS (row-major, stride S). At
+// KC = 256 an FP32 G_S and an FP32 basis do not both fit one CU's LDS, so
+// columns L..255 live in VGPRs, column L + Q m + h of row i as vr[m] on lane
+// (i, h); at KC = 128 the whole basis is in LDS and two subsets share a CU.
+// Gram-Schmidt coefficients of the LDS columns are column dot products (the
+// lanes split into columns x row groups); those of the register columns come
+// from a wavefront reduce-scatter of the lanes' products (wave_rs) and a sum
+// over the wavefronts of each part in LDS. The coefficients are accumulated
+// in T (float, or double: FP64 reorthogonalisation coefficients) and applied
+// in an FP32 update.
+//   KC = 128, Q = 2: 256 lanes, L = 128 (two workgroups per CU)
+//   KC = 256, Q = 2: 512 lanes, L = 140
+//   KC = 256, Q = 4 (1024 lanes, four wavefronts per SIMD) compiles and
+//   agrees with Q = 2, but at 128 VGPRs per lane it spills inside the step
+//   loop and runs at half the speed; it is not offered as a variant.
 // ---------------------------------------------------------------------------
-template <int KC>
+template <int KC, int Q>
 struct HR {
-  static_assert(KC == 128 || KC == 256, "half-row capacity");
-  static constexpr int NT = 2 * KC;             // threads per workgroup
-  static constexpr int HW = KC / 2;             // G_S columns per lane
-  static constexpr int WPH = KC / 64;           // wavefronts per half
-  static constexpr int NW = 2 * WPH;            // wavefronts
+  static_assert((KC == 128 && Q == 2) || (KC == 256 && (Q == 2 || Q == 4)), "row-part layout");
+  static constexpr int NT = Q * KC;             // threads per workgroup
+  static constexpr int HW = KC / Q;             // G_S columns per lane
+  static constexpr int WPH = KC / 64;           // wavefronts per row part
+  static constexpr int NW = Q * WPH;            // wavefronts
   static constexpr int LOGK = KC == 256 ? 8 : 7;
   static constexpr int L = KC == 256 ? 140 : 128;  // basis columns in LDS
   static constexpr int S = KC == 256 ? 140 : 132;  // LDS row stride: 16-byte rows, stride/4 odd
-  static constexpr int R = (KC - L) / 2;        // register basis columns per lane
+  static constexpr int R = (KC - L) / Q;        // register basis columns per lane
   static constexpr int RA = R > 0 ? R : 1;      // their array extent
-  static constexpr int RP = R > 0 ? 64 : 1;     // register partial slots per (half, wavefront)
-  static_assert(R <= 64, "register partial slots");
   static_assert(S % 4 == 0 && (S / 4) % 2 == 1 && L % 4 == 0 && L <= S, "row layout");
 };
 constexpr int DPP_ROW_ROR8 = 0x128;  // lane i <- lane (i + 8) mod 16 within a row
@@ -1015,28 +1019,43 @@ constexpr int k2_chunk() {
   return sizeof(T) == 8 ? 8 : 16;
 }
 
-template <int KC, typename T>
+// LDS work buffers. Members of a union are live in disjoint parts of a
+// Lanczos step, separated by barriers: pw (matvec partials) from the matvec
+// to the barrier after it, rpart from the reduce-scatter to the coefficient
+// sums, part from the update to the sum after the next barrier; lw (the
+// broadcast of w, and the index staging of the gather) until the column dots
+// end, lh from the coefficient sums to the update.
+template <int KC, int Q, typename T>
 struct HRShared {
-  using C = HR<KC>;
+  using C = HR<KC, Q>;
+  static constexpr int RP = C::R > 0 ? (C::R + k2_chunk<T>() - 1) / k2_chunk<T>() * k2_chunk<T>()
+                                     : 1;  // register partial slots per (part, wavefront)
   float lvn[KC];     // next Lanczos vector, unnormalised: the matvec input
-  float lw[KC];      // w broadcast (also the index staging of the gather)
+  union {
+    float lw[KC];    // w broadcast
+    float lh[KC];    // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
+  };
   T lwT[sizeof(T) == 8 ? KC : 4];  // w in T (FP64 coefficients only): one conversion per entry
-  float pw[2][KC];   // matvec partials of the two halves
-  float part[2][KC];  // Gram-Schmidt update partials of the two halves
-  float lh[KC];       // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
-  T cp[C::NT];        // LDS-column dot partials: row group g, column c at g*C + c
-  T rpart[2][C::WPH][C::RP];  // register-column partials per half and wavefront
+  union {
+    float pw[Q][KC];    // matvec partials of the row parts
+    float part[Q][KC];  // Gram-Schmidt update partials of the row parts
+    T rpart[Q][C::WPH][RP];  // register-column partials per part and wavefront
+  };
+  T cp[C::NT];          // LDS-column dot partials: row group g, column c at g * ncl + c
   float ra[2 * C::NW];  // alpha partials (p, e) of all wavefronts
-  float rb[2 * C::WPH];  // beta^2 partials (p, e) of the wavefronts of half 0
+  float rb[2 * C::WPH];  // beta^2 partials (p, e) of the wavefronts of part 0
   float rp[C::WPH];     // ||w||^2 partials before a Gram-Schmidt pass
   float red[16];
   double redd[16];
   int ired[8];
 };
-static_assert(256 * HR<256>::S * sizeof(float) + sizeof(HRShared<256, double>) <= 163840,
-              "LDS budget, capacity 256");
-static_assert(2 * (128 * HR<128>::S * sizeof(float) + sizeof(HRShared<128, double>)) <= 163840,
-              "LDS budget, capacity 128: two workgroups per CU");
+template <int KC, int Q>
+constexpr size_t hr_lds_bytes() {
+  return (size_t)KC * HR<KC, Q>::S * sizeof(float) + sizeof(HRShared<KC, Q, double>);
+}
+static_assert(hr_lds_bytes<256, 2>() <= 163840, "LDS budget, 512 lanes");
+static_assert(hr_lds_bytes<256, 4>() <= 163840, "LDS budget, 1024 lanes");
+static_assert(2 * hr_lds_bytes<128, 2>() <= 163840, "LDS budget, 256 lanes: two workgroups per CU");
 
 template <typename T>
 __device__ __forceinline__ T swap_add(T a, T b, bool wide32);
@@ -1111,10 +1130,10 @@ __device__ __forceinline__ T wave_rs(const T (&x)[N], int q) {
   return r;
 }
 
-// min or max of x over the rows of half 0 (all lanes return it)
-template <bool MAX, int KC, typename T>
-__device__ __forceinline__ double hr_minmax(double x, int slot, HRShared<KC, T> &sh) {
-  constexpr int WPH = HR<KC>::WPH;
+// min or max of x over the rows of part 0 (all lanes return it)
+template <bool MAX, int KC, int Q, typename T>
+__device__ __forceinline__ double hr_minmax(double x, int slot, HRShared<KC, Q, T> &sh) {
+  constexpr int WPH = HR<KC, Q>::WPH;
   x = MAX ? g_max<64>(x) : g_min<64>(x);
   const int t = threadIdx.x;
   __syncthreads();
@@ -1143,11 +1162,13 @@ __device__ __forceinline__ float hr_rowdot(const float (&g)[N], const float *L) 
   return (a0 + a1) + (a2 + a3);
 }
 
-// beta^2 = w.w (compensated, over the rows of half 0) and the next matvec
+// beta^2 = w.w (compensated, over the rows of part 0) and the next matvec
 // input lvn = w, published together behind one barrier
-template <int KC, typename T>
-__device__ __forceinline__ float hr_beta2(float w, HRShared<KC, T> &sh) {
-  const int t = threadIdx.x, i = t & (KC - 1);
+template <int KC, int Q, typename T>
+__device__ __forceinline__ float hr_beta2(float w, HRShared<KC, Q, T> &sh) {
+  int t = threadIdx.x;
+  if constexpr (KC == 256) asm volatile("" : "+v"(t));
+  const int i = t & (KC - 1);
   float p, e;
   g_dot2_pe<64>(w, w, p, e);
   if (t < KC) {
@@ -1160,23 +1181,64 @@ __device__ __forceinline__ float hr_beta2(float w, HRShared<KC, T> &sh) {
   __syncthreads();
   float p0 = sh.rb[0], e0 = sh.rb[1];
 #pragma unroll
-  for (int u = 1; u < HR<KC>::WPH; ++u) comp_step(p0, e0, sh.rb[2 * u], sh.rb[2 * u + 1]);
+  for (int u = 1; u < HR<KC, Q>::WPH; ++u) comp_step(p0, e0, sh.rb[2 * u], sh.rb[2 * u + 1]);
   return p0 + e0;
 }
 
-// One classical Gram-Schmidt pass of w (row i's entry, identical on both
-// halves) against basis columns 0..j; returns the new entry. With WANT_PRE,
+// a / b for 0 <= a < 2^16, 1 <= b < 2^16: float reciprocal estimate, then
+// one correction step each way (the estimate is off by at most one)
+__device__ __forceinline__ int udiv_small(int a, int b) {
+  int q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
+  const int r = a - q * b;
+  q += (r >= b) ? 1 : 0;
+  q -= (r < 0) ? 1 : 0;
+  return q;
+}
+
+// a0..a3 += v[u] * w[r + u] over N rows (N a multiple of 4), w from the
+// broadcast copy in LDS (FP32 lw, or its FP64 copy lwT), accumulators by
+// row mod 4
+template <int N, int KC, int Q, typename T>
+__device__ __forceinline__ void hr_dot_rows(const float (&v)[N], const HRShared<KC, Q, T> &sh,
+                                            int r, T &a0, T &a1, T &a2, T &a3) {
+  if constexpr (sizeof(T) == 8) {
+    const double2 *lw2 = reinterpret_cast<const double2 *>(sh.lwT + r);
+#pragma unroll
+    for (int u = 0; u < N / 4; ++u) {
+      const double2 l0 = lw2[2 * u], l1 = lw2[2 * u + 1];
+      a0 = fmaT((T)v[4 * u + 0], (T)l0.x, a0);
+      a1 = fmaT((T)v[4 * u + 1], (T)l0.y, a1);
+      a2 = fmaT((T)v[4 * u + 2], (T)l1.x, a2);
+      a3 = fmaT((T)v[4 * u + 3], (T)l1.y, a3);
+    }
+  } else {
+    const float4 *lw4 = reinterpret_cast<const float4 *>(sh.lw + r);
+#pragma unroll
+    for (int u = 0; u < N / 4; ++u) {
+      const float4 l = lw4[u];
+      a0 = fmaT((T)v[4 * u + 0], (T)l.x, a0);
+      a1 = fmaT((T)v[4 * u + 1], (T)l.y, a1);
+      a2 = fmaT((T)v[4 * u + 2], (T)l.z, a2);
+      a3 = fmaT((T)v[4 * u + 3], (T)l.w, a3);
+    }
+  }
+}
+
+// One classical Gram-Schmidt pass of w (row i's entry, identical on all row
+// parts) against basis columns 0..j; returns the new entry. With WANT_PRE,
 // ||w||^2 before the pass is returned in pre (its partials ride along with
 // the broadcast of w).
-template <int KC, typename T, bool WANT_PRE>
+template <int KC, int Q, typename T, bool WANT_PRE>
 __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
-                                        const float (&vr)[HR<KC>::RA], HRShared<KC, T> &sh,
-                                        float &pre) {
-  using C = HR<KC>;
+                                        const float (&vr)[HR<KC, Q>::RA],
+                                        HRShared<KC, Q, T> &sh, float &pre) {
+  using C = HR<KC, Q>;
   constexpr int NC = k2_chunk<T>();
   constexpr int SH = NC == 16 ? 2 : 3;
-  const int t = threadIdx.x, i = t & (KC - 1), h = t >> C::LOGK, wv = (t >> 6) & (C::WPH - 1),
-            q = t & 63;
+  // lane coordinates derived here rather than kept live across the step
+  int t = threadIdx.x;
+  if constexpr (KC == 256) asm volatile("" : "+v"(t));
+  const int i = t & (KC - 1), h = t >> C::LOGK, wv = (t >> 6) & (C::WPH - 1), q = t & 63;
   float ws = 0.f;
   if constexpr (WANT_PRE) ws = g_sum_dpp<64>(w * w);
   if (h == 0) {
@@ -1191,58 +1253,47 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     for (int u = 1; u < C::WPH; ++u) s += sh.rp[u];
     pre = s;
   }
-  // (1) LDS columns 0..ncl-1: the 2 KC lanes split into CC columns x 2 KC/CC
-  // row groups of CC/2 rows (CC = 64, 128 or 256, the least that covers
-  // ncl), so every lane works and the dependent chains stay short; rows >= k
-  // are zero
+  // (1) LDS columns 0..ncl-1: lane t takes column t % ncl of row group
+  // t / ncl; ng = min(16, Q KC / ncl) groups of RR rows (RR a multiple of 4,
+  // ng RR >= KC), so nearly every lane works and the dependent chains stay
+  // short; rows >= k are zero
   const int ncl = min(j + 1, C::L);
-  const int lgC = ncl <= 64 ? 6 : ncl <= 128 ? 7 : 8;
-  const int CC = 1 << lgC;
+  const int ng = min(16, udiv_small(C::NT, ncl));
+  const int RR = (udiv_small(KC + ng - 1, ng) + 3) & ~3;
   {
-    const int c = t & (CC - 1), g = t >> lgC;
-    if (c < ncl) {
-      // 16 rows per iteration: their column entries are all loaded before the
-      // FMAs consume them, so each iteration pays the LDS latency once
-      const int RR = CC >> 1;
+    const int g = udiv_small(t, ncl), c = t - g * ncl;
+    if (g < ng) {
+      // RB rows per iteration (16 FP32, 8 FP64: register budget), then 4 at
+      // a time: the column entries of a batch are all loaded before the FMAs
+      // consume them, so each batch pays the LDS latency once
+      constexpr int RB = sizeof(T) == 8 ? 8 : 16;
       const int r0 = g * RR;
-      const int rn = (min(RR, max(0, k - r0)) + 15) & ~15;
+      const int rn = (min(RR, max(0, k - r0)) + 3) & ~3;
       T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-      for (int r = 0; r < rn; r += 16) {
+      int r = 0;
+      for (; r + RB <= rn; r += RB) {
         const float *col = VL + (r0 + r) * C::S + c;
-        float v[16];
+        float v[RB];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = col[u * C::S];
-        if constexpr (sizeof(T) == 8) {
-          const double2 *lw2 = reinterpret_cast<const double2 *>(sh.lwT + r0 + r);
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const double2 l0 = lw2[2 * u], l1 = lw2[2 * u + 1];
-            a0 = fmaT((T)v[4 * u + 0], (T)l0.x, a0);
-            a1 = fmaT((T)v[4 * u + 1], (T)l0.y, a1);
-            a2 = fmaT((T)v[4 * u + 2], (T)l1.x, a2);
-            a3 = fmaT((T)v[4 * u + 3], (T)l1.y, a3);
-          }
-        } else {
-          const float4 *lw4 = reinterpret_cast<const float4 *>(sh.lw + r0 + r);
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const float4 l = lw4[u];
-            a0 = fmaT((T)v[4 * u + 0], (T)l.x, a0);
-            a1 = fmaT((T)v[4 * u + 1], (T)l.y, a1);
-            a2 = fmaT((T)v[4 * u + 2], (T)l.z, a2);
-            a3 = fmaT((T)v[4 * u + 3], (T)l.w, a3);
-          }
-        }
+        for (int u = 0; u < RB; ++u) v[u] = col[u * C::S];
+        hr_dot_rows<RB, KC, Q, T>(v, sh, r0 + r, a0, a1, a2, a3);
       }
-      sh.cp[g * CC + c] = (a0 + a1) + (a2 + a3);
+      for (; r < rn; r += 4) {
+        const float *col = VL + (r0 + r) * C::S + c;
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = col[u * C::S];
+        hr_dot_rows<4, KC, Q, T>(v, sh, r0 + r, a0, a1, a2, a3);
+      }
+      sh.cp[g * ncl + c] = (a0 + a1) + (a2 + a3);
     }
   }
-  // (2) register columns L + 2m + h (m < nm): products reduce-scattered per
+  // (2) register columns L + Q m + h (m < nm): products reduce-scattered per
   // chunk, the chunks holding formed columns only (uniform branch per chunk;
   // unformed columns inside a chunk hold zeros)
   int nm = 0;
   if constexpr (C::R > 0) {
-    nm = (max(0, j + 1 - C::L) + 1) >> 1;
+    nm = (max(0, j + 1 - C::L) + Q - 1) / Q;
     if (nm > 0) {
 #pragma unroll
       for (int c = 0; c < (C::R + NC - 1) / NC; ++c) {
@@ -1259,17 +1310,16 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     }
   }
   __syncthreads();
-  // (3) the coefficients, lane t < KC for column t (zero beyond j); the row
-  // groups' partials are summed in group order
+  // (3) the coefficients, lane t < KC for column t (zero beyond j); the
+  // partials are summed in group (wavefront) order
   if (h == 0) {
     T hc = 0;
     if (i < ncl) {
-      const int ng = C::NT >> lgC;
       hc = sh.cp[i];
-      for (int g = 1; g < ng; ++g) hc += sh.cp[g * CC + i];
+      for (int g = 1; g < ng; ++g) hc += sh.cp[g * ncl + i];
     } else if constexpr (C::R > 0) {
       if (i >= C::L && i <= j) {
-        const int m = (i - C::L) >> 1, hh = (i - C::L) & 1;
+        const int m = (i - C::L) / Q, hh = (i - C::L) % Q;
         T s = sh.rpart[hh][0][m];
 #pragma unroll
         for (int u = 1; u < C::WPH; ++u) s += sh.rpart[hh][u][m];
@@ -1279,17 +1329,17 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     sh.lh[i] = (float)hc;
   }
   __syncthreads();
-  // (4) update partials (FP32): half 0 takes LDS columns [0, cs) and half 1
-  // [cs, ncl), cs = ceil(ncl / 8) * 4; register columns go to their owner half
+  // (4) update partials (FP32): part h takes LDS columns [h cs, (h + 1) cs)
+  // below ncl, cs = ceil(ncl / 4Q) * 4; register columns go to their owner
   float u0 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f;
   {
-    const int cs = ((ncl + 7) >> 3) << 2;
-    const int c0 = h == 0 ? 0 : cs;
-    const int cn = h == 0 ? min(ncl, cs) : max(0, ncl - cs);
+    const int cs = ((ncl + 4 * Q - 1) / (4 * Q)) * 4;
+    const int c0 = h * cs;
+    const int cn = max(0, min(cs, ncl - c0));
     const float *row = VL + i * C::S + c0;
     const float *hl = sh.lh + c0;
     // 8 columns per iteration, then at most one group of 4; reads stay in
-    // [c0, c0 + cn rounded up to 4), inside this half's share of the row and
+    // [c0, c0 + cn rounded up to 4), inside this part's share of the row and
     // below L (columns past j hold zeros and zero coefficients)
     const int cn4 = (cn + 3) & ~3;
     int c = 0;
@@ -1320,21 +1370,24 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
         if (m0 >= nm) break;
 #pragma unroll
         for (int m = m0; m < m0 + 8 && m < C::R; m += 2) {
-          u0 = __builtin_fmaf(vr[m], sh.lh[C::L + 2 * m + h], u0);
-          if (m + 1 < C::R) u1 = __builtin_fmaf(vr[m + 1], sh.lh[C::L + 2 * m + 2 + h], u1);
+          u0 = __builtin_fmaf(vr[m], sh.lh[C::L + Q * m + h], u0);
+          if (m + 1 < C::R) u1 = __builtin_fmaf(vr[m + 1], sh.lh[C::L + Q * (m + 1) + h], u1);
         }
       }
     }
   }
   sh.part[h][i] = (u0 + u1) + (u2 + u3);
   __syncthreads();
-  return w - (sh.part[0][i] + sh.part[1][i]);
+  float tot = sh.part[0][i];
+#pragma unroll
+  for (int hh = 1; hh < Q; ++hh) tot += sh.part[hh][i];
+  return w - tot;
 }
 
-template <int KC, int MODE, typename T>
-__device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShared<KC, T> &sh,
+template <int KC, int Q, int MODE, typename T>
+__device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShared<KC, Q, T> &sh,
                                           int64_t s) {
-  using C = HR<KC>;
+  using C = HR<KC, Q>;
   const int t = threadIdx.x, i = t & (KC - 1), h = t >> C::LOGK, q = t & 63;
   int k;
   int64_t off;
@@ -1418,18 +1471,25 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
       __syncthreads();
       float vp = 0.f, bj = 0.f, amax = 0.f, bmax = 0.f, my_a = 0.f, my_b = 0.f;
       for (int j = 0; j < k; ++j) {
-        // matvec partial over this half's columns; alpha's partial Dot2 of v
-        // with it (the two halves' partials sum to v . G lvn)
-        const float mv = hr_rowdot<C::HW>(g, sh.lvn + C::HW * h);
+        // lane coordinates derived here rather than kept live across the step
+        int tl = threadIdx.x;
+        if constexpr (KC == 256) asm volatile("" : "+v"(tl));
+        const int il = tl & (KC - 1), hl = tl >> C::LOGK;
+        // matvec partial over this part's columns; alpha's partial Dot2 of v
+        // with it (the parts' partials sum to v . G lvn)
+        const float mv = hr_rowdot<C::HW>(g, sh.lvn + C::HW * hl);
         float p, e;
         g_dot2_pe<64>(vi, mv, p, e);
-        sh.pw[h][i] = mv;
-        if (q == 0) {
-          sh.ra[2 * (t >> 6)] = p;
-          sh.ra[2 * (t >> 6) + 1] = e;
+        sh.pw[hl][il] = mv;
+        if ((tl & 63) == 0) {
+          sh.ra[2 * (tl >> 6)] = p;
+          sh.ra[2 * (tl >> 6) + 1] = e;
         }
         __syncthreads();
-        float w = (sh.pw[0][i] + sh.pw[1][i]) * scale;
+        float w = sh.pw[0][il];
+#pragma unroll
+        for (int hh = 1; hh < Q; ++hh) w += sh.pw[hh][il];
+        w *= scale;
         float p0 = sh.ra[0], e0 = sh.ra[1];
 #pragma unroll
         for (int w2 = 1; w2 < C::NW; ++w2) comp_step(p0, e0, sh.ra[2 * w2], sh.ra[2 * w2 + 1]);
@@ -1439,10 +1499,10 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
         if (j == k - 1) break;
         w = __builtin_fmaf(-al, vi, __builtin_fmaf(-bj, vp, w));
         float pre = 0.f, unused;
-        w = hr_cgs<KC, T, true>(w, j, k, VL, vr, sh, pre);
+        w = hr_cgs<KC, Q, T, true>(w, j, k, VL, vr, sh, pre);
         float b2 = hr_beta2(w, sh);
         if (b2 < 0.5f * pre) {  // DGKS: w was mostly in span(V); orthogonalise again
-          w = hr_cgs<KC, T, false>(w, j, k, VL, vr, sh, unused);
+          w = hr_cgs<KC, Q, T, false>(w, j, k, VL, vr, sh, unused);
           b2 = hr_beta2(w, sh);
         }
         const float b = sqrtf(b2);
@@ -1467,7 +1527,10 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
           __syncthreads();
           sh.pw[h][i] = rs;
           __syncthreads();
-          const float res = 1.f - (sh.pw[0][i] + sh.pw[1][i]);
+          float rsum = sh.pw[0][i];
+#pragma unroll
+          for (int hh = 1; hh < Q; ++hh) rsum += sh.pw[hh][i];
+          const float res = 1.f - rsum;
           const int wb = restart_lane<64>(res, i, k);
           const float wvv = __shfl(i < k ? res : -1.f, wb & 63, 64);
           if (t < KC && q == 0) {
@@ -1483,18 +1546,18 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
               bi = sh.ired[w2];
             }
           w = (i == bi) ? 1.f : 0.f;
-          w = hr_cgs<KC, T, false>(w, j, k, VL, vr, sh, unused);
-          w = hr_cgs<KC, T, false>(w, j, k, VL, vr, sh, unused);
+          w = hr_cgs<KC, Q, T, false>(w, j, k, VL, vr, sh, unused);
+          w = hr_cgs<KC, Q, T, false>(w, j, k, VL, vr, sh, unused);
           scale = 1.0f / sqrtf(hr_beta2(w, sh));
         }
         vi = w * scale;
-        // basis column j + 1: LDS, or register slot (j + 1 - L) / 2 of one half
+        // basis column j + 1: LDS, or register slot (j + 1 - L) / Q of one part
         const int c = j + 1;
         if (c < C::L) {
           if (h == 0) VL[i * C::S + c] = vi;
         } else if constexpr (C::R > 0) {
-          const int m = (c - C::L) >> 1;
-          const bool mine = ((c - C::L) & 1) == h;
+          const int m = (c - C::L) / Q;
+          const bool mine = ((c - C::L) % Q) == h;
 #pragma unroll
           for (int r = 0; r < C::R; ++r) vr[r] = (mine && r == m) ? vi : vr[r];
         }
@@ -1504,7 +1567,7 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
       if (__syncthreads_or(!finite)) st |= ST_NONFINITE;
       if (!(st & FATAL)) {
         float2 *tri = a.tri + s * (int64_t)a.tstride;
-        sh.pw[0][i] = my_b;  // both halves write the same value
+        sh.pw[0][i] = my_b;  // every part writes the same value
         __syncthreads();
         const double al = my_a, b = my_b, bn = (i + 1 < KC) ? (double)sh.pw[0][i + 1] : 0.0;
         double glo = INFINITY, ghi = -INFINITY, amin = INFINITY, amx = -INFINITY;
@@ -1526,10 +1589,10 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
   }
 }
 
-template <int KC, int MODE, typename T>
-__global__ __launch_bounds__(2 * KC) void solve_halfrow_kernel(SolveArgs a) {
-  __shared__ __attribute__((aligned(16))) float VL[KC * HR<KC>::S];
-  __shared__ __attribute__((aligned(16))) HRShared<KC, T> sh;
+template <int KC, int Q, int MODE, typename T>
+__global__ __launch_bounds__(Q * KC) void solve_rowpart_kernel(SolveArgs a) {
+  __shared__ __attribute__((aligned(16))) float VL[KC * HR<KC, Q>::S];
+  __shared__ __attribute__((aligned(16))) HRShared<KC, Q, T> sh;
   int64_t cnt = a.nslots;
   const int32_t *ord = a.order;
   if (a.bins) {
@@ -1537,7 +1600,7 @@ __global__ __launch_bounds__(2 * KC) void solve_halfrow_kernel(SolveArgs a) {
     ord = a.order + a.bins[BIN_START + a.bin];
   }
   for (int64_t slot = blockIdx.x; slot < cnt; slot += gridDim.x) {
-    hr_subset<KC, MODE, T>(a, VL, sh, ord ? (int64_t)ord[slot] : slot);
+    hr_subset<KC, Q, MODE, T>(a, VL, sh, ord ? (int64_t)ord[slot] : slot);
     __syncthreads();
   }
 }
@@ -1550,19 +1613,20 @@ struct VariantFns {
                                      solve_kernel<GROUP, PAD, 2>};
 };
 
-template <int KC>
-static SolveFn pick_halfrow(int mode, bool f64) {
-  static constexpr SolveFn f32[3] = {solve_halfrow_kernel<KC, 0, float>,
-                                     solve_halfrow_kernel<KC, 1, float>,
-                                     solve_halfrow_kernel<KC, 2, float>};
-  return (f64 && mode == MODE_TRIDIAG) ? solve_halfrow_kernel<KC, 0, double> : f32[mode];
+template <int KC, int Q>
+static SolveFn pick_rowpart(int mode, bool f64) {
+  static constexpr SolveFn f32[3] = {solve_rowpart_kernel<KC, Q, 0, float>,
+                                     solve_rowpart_kernel<KC, Q, 1, float>,
+                                     solve_rowpart_kernel<KC, Q, 2, float>};
+  return (f64 && mode == MODE_TRIDIAG) ? solve_rowpart_kernel<KC, Q, 0, double> : f32[mode];
 }
 
 // group = lanes per subset: 8..64 (lane groups), 128 (one row per lane),
-// 256 and 512 (half rows, capacities 128 and 256)
+// 256 and 512 (half rows: capacities 128 and 256)
 static SolveFn pick(int group, int pad, int mode, bool f64) {
-  if (group == 2 * 256) return pad == 4 ? pick_halfrow<256>(mode, f64) : nullptr;
-  if (group == 2 * 128) return pad == 4 ? pick_halfrow<128>(mode, f64) : nullptr;
+  if (group >= 256 && pad != 4) return nullptr;
+  if (group == 512) return pick_rowpart<256, 2>(mode, f64);
+  if (group == 256) return pick_rowpart<128, 2>(mode, f64);
   if (group == WIDE) {
     static constexpr SolveFn w1[3] = {solve_wide_kernel<1, 0>, solve_wide_kernel<1, 1>,
                                       solve_wide_kernel<1, 2>};

@@ -119,9 +119,9 @@ def test_variants_all_k(gpu, k):
     olo, ohi, _, ost = oracle.solve_batch(Gh, ih, k=k)
     refs = {}
     for lanes in (8, 16, 32, 64, 128, 256, 512):
-        # lanes per subset; the half-row kernels (256 and 512 lanes) take
-        # k <= lanes / 2
-        if (lanes if lanes <= 128 else lanes // 2) < k:
+        # lanes per subset; the half-row kernels take k <= 128 (256 lanes) and
+        # k <= 256 (512 lanes)
+        if {256: 128, 512: 256}.get(lanes, lanes) < k:
             continue
         for pad in ((0, 1, 4) if lanes < 128 else (1, 4) if lanes == 128 else (4,)):
             lo, hi, de, st = [t.cpu().numpy() for t in solver.solve(G, idx, k=k, lanes=lanes, pad=pad)]
