@@ -415,7 +415,7 @@ def main():
                      "nonzero_status": int(np.count_nonzero(status))},
         "roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "solve_halfrow_kernel<256, TRIDIAG, double>",
+                     "kernel": "solve_rowpart_kernel<256, 2, TRIDIAG, double>",
                      "launch_ms": dom_ms, "subsets_per_launch": n256},
         "roofline_hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS},

@@ -1042,8 +1042,8 @@ struct HRShared {
     T rpart[Q][C::WPH][RP];  // register-column partials per part and wavefront
   };
   T cp[C::NT];          // LDS-column dot partials: row group g, column c at g * ncl + c
-  float ra[2 * C::NW];  // alpha partials (p, e) of all wavefronts
-  float rb[2 * C::WPH];  // beta^2 partials (p, e) of the wavefronts of part 0
+  double ra[C::NW];     // alpha partials (FP64) of all wavefronts
+  double rb[C::WPH];    // beta^2 partials (FP64) of the wavefronts of part 0
   float rp[C::WPH];     // ||w||^2 partials before a Gram-Schmidt pass
   float red[16];
   double redd[16];
@@ -1162,27 +1162,37 @@ __device__ __forceinline__ float hr_rowdot(const float (&g)[N], const float *L) 
   return (a0 + a1) + (a2 + a3);
 }
 
-// beta^2 = w.w (compensated, over the rows of part 0) and the next matvec
-// input lvn = w, published together behind one barrier
+// FP64 sum over the 64 lanes of a wavefront (every lane gets it). Fed with
+// exact products of FP32 operands, it gives the extended-precision dot
+// product that compensated FP32 summation (Dot2) emulates, in fewer
+// instructions: FP64 is native here.
+__device__ __forceinline__ double wave_sum_f64(double x) {
+  x += dpp<DPP_QUAD_1032>(x);
+  x += dpp<DPP_QUAD_2301>(x);
+  x += dpp<DPP_ROW_HMIRROR>(x);
+  x += dpp<DPP_ROW_MIRROR>(x);
+  x = swap_add<double>(x, x, false);
+  x = swap_add<double>(x, x, true);
+  return x;
+}
+
+// beta^2 = w.w (exact squares summed in FP64, over the rows of part 0) and
+// the next matvec input lvn = w, published together behind one barrier
 template <int KC, int Q, typename T>
 __device__ __forceinline__ float hr_beta2(float w, HRShared<KC, Q, T> &sh) {
   int t = threadIdx.x;
   if constexpr (KC == 256) asm volatile("" : "+v"(t));
   const int i = t & (KC - 1);
-  float p, e;
-  g_dot2_pe<64>(w, w, p, e);
+  const double sd = wave_sum_f64((double)w * (double)w);
   if (t < KC) {
     sh.lvn[i] = w;
-    if ((t & 63) == 0) {
-      sh.rb[2 * (t >> 6)] = p;
-      sh.rb[2 * (t >> 6) + 1] = e;
-    }
+    if ((t & 63) == 0) sh.rb[t >> 6] = sd;
   }
   __syncthreads();
-  float p0 = sh.rb[0], e0 = sh.rb[1];
+  double tot = sh.rb[0];
 #pragma unroll
-  for (int u = 1; u < HR<KC, Q>::WPH; ++u) comp_step(p0, e0, sh.rb[2 * u], sh.rb[2 * u + 1]);
-  return p0 + e0;
+  for (int u = 1; u < HR<KC, Q>::WPH; ++u) tot += sh.rb[u];
+  return (float)tot;
 }
 
 // a / b for 0 <= a < 2^16, 1 <= b < 2^16: float reciprocal estimate, then
@@ -1388,7 +1398,11 @@ template <int KC, int Q, int MODE, typename T>
 __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShared<KC, Q, T> &sh,
                                           int64_t s) {
   using C = HR<KC, Q>;
-  const int t = threadIdx.x, i = t & (KC - 1), h = t >> C::LOGK, q = t & 63;
+  // lane coordinates kept opaque per subset, so that nothing derived from
+  // them is hoisted out of the persistent loop and held live across it
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  const int i = t & (KC - 1), h = t >> C::LOGK, q = t & 63;
   int k;
   int64_t off;
   if (a.offsets) {
@@ -1454,9 +1468,9 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
       // ---- (b) Lanczos recurrence, ParaLanczos.cu:1285-1374 with the build's
       // substitutions (ones/sqrt(k) start, full reorthogonalisation, restart).
       // Each step: matvec of the unnormalised vector published in lvn, scaled
-      // by 1/beta, with alpha's compensated partials exchanged alongside;
-      // three-term update; Gram-Schmidt pass (||w||^2 rides along); beta^2
-      // published with the next vector. Six barriers per step. ----
+      // by 1/beta, with alpha's FP64 partials exchanged alongside; three-term
+      // update; Gram-Schmidt pass (||w||^2 rides along); beta^2 published
+      // with the next vector. Six barriers per step. ----
       float vr[C::RA];
 #pragma unroll
       for (int m = 0; m < C::RA; ++m) vr[m] = 0.f;
@@ -1475,25 +1489,22 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
         int tl = threadIdx.x;
         if constexpr (KC == 256) asm volatile("" : "+v"(tl));
         const int il = tl & (KC - 1), hl = tl >> C::LOGK;
-        // matvec partial over this part's columns; alpha's partial Dot2 of v
-        // with it (the parts' partials sum to v . G lvn)
+        // matvec partial over this part's columns; alpha's partial over it:
+        // exact products v_i mv_i summed in FP64 (the parts' partials sum to
+        // v . G lvn)
         const float mv = hr_rowdot<C::HW>(g, sh.lvn + C::HW * hl);
-        float p, e;
-        g_dot2_pe<64>(vi, mv, p, e);
+        const double pa = wave_sum_f64((double)vi * (double)mv);
         sh.pw[hl][il] = mv;
-        if ((tl & 63) == 0) {
-          sh.ra[2 * (tl >> 6)] = p;
-          sh.ra[2 * (tl >> 6) + 1] = e;
-        }
+        if ((tl & 63) == 0) sh.ra[tl >> 6] = pa;
         __syncthreads();
         float w = sh.pw[0][il];
 #pragma unroll
         for (int hh = 1; hh < Q; ++hh) w += sh.pw[hh][il];
         w *= scale;
-        float p0 = sh.ra[0], e0 = sh.ra[1];
+        double asum = sh.ra[0];
 #pragma unroll
-        for (int w2 = 1; w2 < C::NW; ++w2) comp_step(p0, e0, sh.ra[2 * w2], sh.ra[2 * w2 + 1]);
-        const float al = (p0 + e0) * scale;
+        for (int w2 = 1; w2 < C::NW; ++w2) asum += sh.ra[w2];
+        const float al = (float)(asum * (double)scale);
         my_a = (i == j) ? al : my_a;
         amax = fmaxf(amax, fabsf(al));
         if (j == k - 1) break;
