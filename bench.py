@@ -23,6 +23,8 @@ Reported beside it, in the same JSON line:
     FP64 Gram-Schmidt coefficients), timed with HIP events on its stream;
   * the gather stage alone over the same subsets: algorithmic GB/s and the
     HBM fetch rate the profiler's counters give for it (profiles/);
+  * gram: G = Phi^T Phi of the serving frame by pl_gram (matrix cores) against
+    one FP32 library GEMM of the full product (set-up, outside the timing);
   * cpu_baseline: the FP64 oracle on the box's host cores over a bounded
     sample of the same subsets;
   * batching: the mixed k in {8..128} configuration (tuned binned persistent
@@ -179,6 +181,42 @@ def precision_leg(solver, torch, dev, sample=1024):
     return res
 
 
+def gram_leg(solver, torch, phi, G):
+    """G = Phi^T Phi of the serving frame (m = 4096, n = 32768, column layout):
+    pl_gram (matrix cores, upper-triangle tiles mirrored) against one FP32
+    library GEMM of the full product, HIP events on torch's stream."""
+    n, m = phi.shape
+    tiles = -(-n // 128)
+    flops = tiles * (tiles + 1) // 2 * 2.0 * 128 * 128 * m  # computed tiles
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+
+    def timed(f, reps):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = ev(), ev()
+        e0.record()
+        for _ in range(reps):
+            out = f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, out
+
+    ms, _ = timed(lambda: solver.gram(phi), 3)
+    prev = torch.backends.cuda.matmul.allow_tf32
+    torch.backends.cuda.matmul.allow_tf32 = False
+    try:
+        lib_ms, Gl = timed(lambda: phi @ phi.T, 2)
+    finally:
+        torch.backends.cuda.matmul.allow_tf32 = prev
+    diff = (Gl - G).abs().max().item()
+    del Gl
+    torch.cuda.empty_cache()
+    return {"workload": f"G = Phi^T Phi, Phi {m} x {n} FP32 (column layout), G {n} x {n}",
+            "ms": ms, "tflops": flops / (ms / 1e3) / 1e12, "peak_tflops": FP32_PEAK_TFLOPS,
+            "frac": flops / (ms / 1e3) / 1e12 / FP32_PEAK_TFLOPS, "flops_per_launch": flops,
+            "library_gemm_ms": lib_ms, "vs_library": lib_ms / ms, "max_abs_diff_vs_library": diff}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +235,8 @@ def main():
                     help="skip the batching and precision configurations")
     ap.add_argument("--profile-gather", action="store_true",
                     help="only run the gather stage over the dominant subsets (rocprof counters)")
+    ap.add_argument("--profile-gram", action="store_true",
+                    help="run pl_gram on the serving frame 1 + 3 times and exit (for rocprofv3)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy this many MiB device to device and exit (calibrates the "
                          "profiler's byte counters against a transfer of known size)")
@@ -222,7 +262,16 @@ def main():
         return 0
     t_setup = time.perf_counter()
     phi = solver.gen_phi("dct", SEED_PHI, M, N)
+    if args.profile_gram:
+        G = solver.gram(phi)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            solver.gram(phi)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_gram": True, "m": M, "n": N}), flush=True)
+        return 0
     G = solver.gram(phi)
+    gram = None if (args.profile_dominant or args.profile_gather) else gram_leg(solver, torch, phi, G)
     del phi
     torch.cuda.empty_cache()
     reqs = make_requests(solver, torch, args.requests)
@@ -420,6 +469,7 @@ def main():
         "roofline_hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS},
         "gather": gather,
+        "gram": gram,
         **legs,
         "cpu_baseline": cpu,
         "setup_s": time.perf_counter() - t_setup - wall,

@@ -239,6 +239,19 @@ pl_status pl_gen_subsets_csr(uint64_t seed, int64_t s0, int64_t nsub, const int6
 pl_status pl_gen_ks(uint64_t seed, int64_t s0, int64_t nsub, const int32_t *klist, int nk,
                     int32_t *ks, void *stream);
 
+/* Gram matrix G = Phi^T Phi (n x n FP32, row-major, leading dimension ldg >= n)
+   of an m x n FP32 frame Phi, on the matrix cores (FP32 products, FP32
+   accumulation). Replaces the reference's entry-by-entry Gram builders DSG_B /
+   DSG_B1 / SG_B1 (ParaLanczos.cu:364-438), which take the frame row-major,
+   Frame[r*N + c]. Layouts: PL_LAYOUT_ROW, entry (r, c) at phi[r*ld + c]
+   (ld >= n; the reference's), or PL_LAYOUT_COL, entry (r, c) at phi[c*ld + r]
+   (ld >= m; the layout pl_gen_phi writes). Only upper-triangle tiles are
+   computed; G is written whole and exactly symmetric. Asynchronous on stream. */
+#define PL_LAYOUT_ROW 0
+#define PL_LAYOUT_COL 1
+pl_status pl_gram(const float *phi, int64_t m, int64_t n, int64_t ld, int layout, float *G,
+                  int64_t ldg, void *stream);
+
 /* synthetic sensing matrix, column-major m x n (phi[j*m + r]):
    kind 0: N(0,1/m) entries, columns normalised to unit L2 norm;
    kind 1: +-1/sqrt(m) equiprobable;

@@ -16,7 +16,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libparalanczos.so")
-SOURCES = ["pl_kernels.hip", "pl_capi.hip", "pl_server.hip"]
+SOURCES = ["pl_kernels.hip", "pl_capi.hip", "pl_server.hip", "pl_gram.hip"]
 ARCH = "gfx950"
 
 

@@ -85,16 +85,25 @@ def gen_phi(kind: str, seed: int, m: int, n: int, device="cuda"):
     return phi
 
 
-def gram(phiT: torch.Tensor) -> torch.Tensor:
-    """G = Phi^T Phi in FP32 (a plain library GEMM; no TF32/XF32), made exactly
-    symmetric by averaging with its transpose."""
-    prev = torch.backends.cuda.matmul.allow_tf32
-    torch.backends.cuda.matmul.allow_tf32 = False
-    try:
-        G = phiT @ phiT.T
-    finally:
-        torch.backends.cuda.matmul.allow_tf32 = prev
-    G.add_(G.T.contiguous()).mul_(0.5)
+LAYOUTS = {"row": 0, "col": 1}
+
+
+def gram(phi: torch.Tensor, layout: str = "col") -> torch.Tensor:
+    """G = Phi^T Phi (FP32, exactly symmetric) on the matrix cores (pl_gram).
+    layout 'col': phi is the (n, m) tensor gen_phi returns (row j = column j
+    of Phi); 'row': phi is Phi itself, (m, n), as the reference's frame."""
+    L = load()
+    _dev(phi, torch.float32, "phi")
+    if phi.dim() != 2 or phi.stride(1) != 1:
+        raise ParaLanczosError("phi must be a 2-D tensor with unit column stride")
+    if layout == "col":
+        n, m = phi.shape
+    elif layout == "row":
+        m, n = phi.shape
+    else:
+        raise ParaLanczosError("layout must be 'row' or 'col'")
+    G = torch.empty((n, n), dtype=torch.float32, device=phi.device)
+    check(L.pl_gram(_p(phi), m, n, phi.stride(0), LAYOUTS[layout], _p(G), n, _stream()), "pl_gram")
     return G
 
 
