@@ -272,7 +272,6 @@ def main():
         return 0
     G = solver.gram(phi)
     gram = None if (args.profile_dominant or args.profile_gather) else gram_leg(solver, torch, phi, G)
-    del phi
     torch.cuda.empty_cache()
     reqs = make_requests(solver, torch, args.requests)
     tot = sum(q["nsub"] for q in reqs)
@@ -282,6 +281,22 @@ def main():
     # k = 256 subsets as one step holds, on torch's current stream
     n256 = int(sum(int((q["ks"] == DOMINANT_K).sum().item()) for q in reqs))
     idx256 = solver.gen_subsets(SEED_SAMPLE, n256, DOMINANT_K, N)
+    if gram is not None:
+        # the same subsets' G_S straight from the frame (pl_subgram), against
+        # gathering them from the resident G
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        sg = solver.subgram(phi, idx256, DOMINANT_K)
+        del sg  # its blocks are reused from torch's cache by the timed call
+        torch.cuda.synchronize()
+        e0.record()
+        sg = solver.subgram(phi, idx256, DOMINANT_K)
+        e1.record()
+        torch.cuda.synchronize()
+        gram["subgram_k256_ms"] = e0.elapsed_time(e1)
+        gram["subgram_subsets"] = n256
+        del sg
+    del phi
+    torch.cuda.empty_cache()
     ws = torch.empty(solver.workspace_size(n256, DOMINANT_K), dtype=torch.uint8, device=dev)
     o256 = solver.solve_stage(G, idx256, 1, ws, k=DOMINANT_K)
     torch.cuda.synchronize()

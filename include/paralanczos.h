@@ -252,6 +252,19 @@ pl_status pl_gen_ks(uint64_t seed, int64_t s0, int64_t nsub, const int32_t *klis
 pl_status pl_gram(const float *phi, int64_t m, int64_t n, int64_t ld, int layout, float *G,
                   int64_t ldg, void *stream);
 
+/* Sub-Gram matrices straight from the frame, the reference's DSG_B form
+   (ParaLanczos.cu:364-382, subgrams[j + i*n] = sum_k Frame[subset[i] + k*N]
+   * Frame[subset[j] + k*N]) over a batch: out[s*k*k + r*k + c] =
+   sum_t Phi(t, idx[s*k + r]) Phi(t, idx[s*k + c]), uniform k in [1, 256],
+   on the matrix cores with the accumulation order of pl_gram (so each G_S
+   equals the one pl_gather takes from pl_gram's G, bit for bit). An
+   alternative to gathering from a materialised G. Frame layouts as pl_gram.
+   status (nsub entries, may be NULL): 0, or PL_SUB_BADINDEX for a subset with
+   an index outside [0, n), whose output is left unwritten. */
+pl_status pl_subgram(const float *phi, int64_t m, int64_t n, int64_t ld, int layout,
+                     const int32_t *idx, int k, int64_t nsub, float *out, uint32_t *status,
+                     void *stream);
+
 /* synthetic sensing matrix, column-major m x n (phi[j*m + r]):
    kind 0: N(0,1/m) entries, columns normalised to unit L2 norm;
    kind 1: +-1/sqrt(m) equiprobable;

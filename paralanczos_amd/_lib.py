@@ -50,6 +50,7 @@ SIGNATURES = {
     "pl_gen_ks": (ST, [U64, I64, I64, P, INT, P, P]),
     "pl_gen_phi": (ST, [INT, U64, INT, INT, P, P]),
     "pl_gram": (ST, [P, I64, I64, I64, INT, P, I64, P]),
+    "pl_subgram": (ST, [P, I64, I64, I64, INT, P, INT, I64, P, P, P]),
     # ParaLanczos.h
     "print_tridiagonal": (None, [P, P, INT]),
     "calculate_eigs": (None, [P, INT, P, P]),

@@ -107,6 +107,28 @@ def gram(phi: torch.Tensor, layout: str = "col") -> torch.Tensor:
     return G
 
 
+def subgram(phi: torch.Tensor, idx: torch.Tensor, k: int, layout: str = "col"):
+    """G_S = Phi_S^T Phi_S of every index set S (uniform k) straight from the
+    frame (pl_subgram): returns (nsub, k, k) FP32 and the int32 status view."""
+    L = load()
+    _dev(phi, torch.float32, "phi")
+    _dev(idx, torch.int32, "idx")
+    if phi.dim() != 2 or phi.stride(1) != 1:
+        raise ParaLanczosError("phi must be a 2-D tensor with unit column stride")
+    if layout == "col":
+        n, m = phi.shape
+    elif layout == "row":
+        m, n = phi.shape
+    else:
+        raise ParaLanczosError("layout must be 'row' or 'col'")
+    nsub = idx.numel() // k
+    out = torch.empty((nsub, k, k), dtype=torch.float32, device=phi.device)
+    status = torch.empty(nsub, dtype=torch.int32, device=phi.device)
+    check(L.pl_subgram(_p(phi), m, n, phi.stride(0), LAYOUTS[layout], _p(idx), int(k), nsub,
+                       _p(out), _p(status), _stream()), "pl_subgram")
+    return out, status
+
+
 def solve(G, idx, k=None, offsets=None, form="tuned", lanes=None, pad=None, out=None,
           reorth=None):
     """lambda_min, lambda_max, delta (FP32) and status (int32 view of the u32

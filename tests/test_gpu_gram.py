@@ -1,5 +1,5 @@
-"""pl_gram: G = Phi^T Phi on the matrix cores, against FP64 products of the
-same FP32 frame.
+"""pl_gram (G = Phi^T Phi) and pl_subgram (G_S = Phi_S^T Phi_S per index set)
+on the matrix cores, against FP64 products of the same FP32 frame.
 
 The bound is the classical one for a sum of m products in FP32, whatever the
 order: |G_ij - (Phi^T Phi)_ij| <= gamma_{m+1} (|Phi|^T |Phi|)_ij with
@@ -85,3 +85,37 @@ def test_gram_argument_errors(gpu):
     assert L.pl_gram(P, 4, 2, 3, 1, P, 2, None) == 1   # column layout: ld < m
     assert L.pl_gram(P, 4, 4, 4, 2, P, 4, None) == 1   # unknown layout
     assert L.pl_gram(P, 4, 4, 4, 0, P, 3, None) == 1   # ldg < n
+
+
+@pytest.mark.parametrize("k", [1, 5, 32, 64, 65, 128, 200, 256])
+@pytest.mark.parametrize("layout", ["row", "col"])
+def test_subgram_vs_fp64_and_gram(gpu, k, layout):
+    """Every sub-Gram meets the FP32 bound against FP64 and equals, bit for
+    bit, the G_S that pl_gather takes from pl_gram's G (same accumulation
+    order over the frame's rows)."""
+    m, n, nsub = 300, 700, 12
+    g = torch.Generator(device="cuda").manual_seed(77 + k)
+    Phi = torch.randn((m, n), device="cuda", generator=g, dtype=torch.float32)
+    src = Phi if layout == "row" else Phi.T.contiguous()
+    idx = solver.gen_subsets(900 + k, nsub, k, n)
+    out, st = solver.subgram(src, idx, k, layout=layout)
+    out = out.cpu().numpy()
+    assert np.all(st.cpu().numpy() == 0)
+    P64 = Phi.cpu().numpy().astype(np.float64)
+    ih = idx.cpu().numpy().reshape(nsub, k)
+    for s in range(nsub):
+        _check(out[s], P64[:, ih[s]])
+    G = solver.gram(src, layout=layout)
+    via_g = solver.gather(G, idx, k, "dump", 0, 4).cpu().numpy().reshape(nsub, k, k)
+    assert np.array_equal(out, via_g)
+
+
+def test_subgram_bad_index_flagged(gpu):
+    m, n, k = 64, 100, 16
+    g = torch.Generator(device="cuda").manual_seed(3)
+    phiT = torch.randn((n, m), device="cuda", generator=g, dtype=torch.float32)
+    idx = solver.gen_subsets(4, 3, k, n)
+    idx[k + 5] = n  # subset 1 reaches past the frame
+    out, st = solver.subgram(phiT, idx, k)
+    st = st.cpu().numpy()
+    assert st[0] == 0 and st[2] == 0 and st[1] == 8
