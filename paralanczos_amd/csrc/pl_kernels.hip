@@ -1893,97 +1893,52 @@ __device__ __forceinline__ void stage_tridiag(const BisectArgs &a, int64_t s0, i
   __syncthreads();
 }
 
-// Sturm counts of the same T at two shifts, interleaved so that the two
-// independent recurrences share the LDS reads and fill each other's latency
-__device__ __forceinline__ void sturm_count2(const float2 *tri, int k, double sa, double sb,
-                                             int &ca, int &cb) {
-  double a0 = 1.0, b0 = 1.0;
-  double a1 = (double)tri[0].x - sa;
-  double b1 = (double)tri[0].x - sb;
-  if (a1 == 0.0) a1 = -0x1p-60;
-  if (b1 == 0.0) b1 = -0x1p-60;
-  bool an = a1 < 0.0, bn = b1 < 0.0;
-  ca = an;
-  cb = bn;
-  float2 t = tri[k > 1 ? 1 : 0];
-  for (int i = 1; i < k; ++i) {
-    const float2 tn = tri[i + 1 < k ? i + 1 : i];
-    const double al = (double)t.x;
-    const double bt = (double)t.y;
-    const double b2 = bt * bt;
-    double a2 = __builtin_fma(al - sa, a1, -b2 * a0);
-    double b2s = __builtin_fma(al - sb, b1, -b2 * b0);
-    a2 = (a2 == 0.0) ? -a1 * 0x1p-60 : a2;
-    b2s = (b2s == 0.0) ? -b1 * 0x1p-60 : b2s;
-    t = tn;
-    const bool an2 = a2 < 0.0, bn2 = b2s < 0.0;
-    ca += an2 != an;
-    cb += bn2 != bn;
-    an = an2;
-    bn = bn2;
-    a0 = a1;
-    a1 = a2;
-    b0 = b1;
-    b1 = b2s;
-    if ((i & 7) == 0) {
-      int ea, eb;
-      (void)frexp(fmax(fabs(a0), fabs(a1)), &ea);
-      (void)frexp(fmax(fabs(b0), fabs(b1)), &eb);
-      a0 = ldexp(a0, -ea);
-      a1 = ldexp(a1, -ea);
-      b0 = ldexp(b0, -eb);
-      b1 = ldexp(b1, -eb);
-    }
-  }
-}
-
-// lambda_min / lambda_max: lane j of a block bisects both extremes of subset j
-// at once; brackets [Gershgorin lo, min alpha] and [max alpha, Gershgorin hi]
-// hold since lambda_min(T) <= T_ii <= lambda_max(T)
+// lambda_min / lambda_max: lanes 2j and 2j + 1 of a block bisect lambda_min
+// and lambda_max of the block's subset j; brackets [Gershgorin lo, min alpha]
+// and [max alpha, Gershgorin hi] hold since lambda_min(T) <= T_ii <=
+// lambda_max(T). Each extreme follows its own midpoint sequence and stops at
+// its own width, so the two lanes finish independently; the results meet in
+// lane 2j.
+constexpr int BIS_SUBS = BIS_BLOCK / 2;  // subsets per block
 __global__ __launch_bounds__(BIS_BLOCK) void bisect_extreme_kernel(BisectArgs a) {
   extern __shared__ float2 stage[];
-  const int64_t s0 = (int64_t)blockIdx.x * BIS_BLOCK;
-  const int cnt = (int)min<int64_t>(BIS_BLOCK, a.nsub - s0);
+  const int64_t s0 = (int64_t)blockIdx.x * BIS_SUBS;
+  const int cnt = (int)min<int64_t>(BIS_SUBS, a.nsub - s0);
   stage_tridiag(a, s0, cnt, stage);
-  const int j = threadIdx.x;
-  if (j >= cnt) return;
+  const int j = threadIdx.x >> 1, upper = threadIdx.x & 1;
+  if (j >= cnt) return;  // both lanes of a pair leave together
   const int64_t s = subset_at(a, s0 + j);
   const int k = subset_k(a, s);
-  uint32_t st = a.status[s];
-  double lmin = NAN, lmax = NAN;
+  const uint32_t st = a.status[s];
+  double lam = NAN;
+  int noconv = 0;
   if (!(st & (ST_BADK | ST_BADINDEX | ST_NONFINITE))) {
     const float2 *tri = stage + j * (a.tstride + 1);
     const double4 bd = a.bounds[s];
     double glo, ghi, tnorm;
     widened(bd, k, glo, ghi, tnorm);
     const double floor_tol = ldexp(tnorm, -52) + 0x1p-1000;
-    double lo0 = glo, hi0 = bd.z, lo1 = bd.w, hi1 = ghi;
+    double lo = upper ? bd.w : glo, hi = upper ? ghi : bd.z;
+    const int target = upper ? k - 1 : 0;
     int it = 0;
     for (; it < BIS_MAXIT; ++it) {
-      const double tol0 = fmax(ldexp(fmax(fabs(lo0), fabs(hi0)), -BIS_TOLBITS_EXT), floor_tol);
-      const double tol1 = fmax(ldexp(fmax(fabs(lo1), fabs(hi1)), -BIS_TOLBITS_EXT), floor_tol);
-      const bool d0 = hi0 - lo0 <= tol0, d1 = hi1 - lo1 <= tol1;
-      if (d0 && d1) break;
-      const double m0 = lo0 + 0.5 * (hi0 - lo0), m1 = lo1 + 0.5 * (hi1 - lo1);
-      int c0, c1;
-      sturm_count2(tri, k, m0, m1, c0, c1);
-      if (!d0) {
-        if (c0 > 0) hi0 = m0;
-        else lo0 = m0;
-      }
-      if (!d1) {
-        if (c1 > k - 1) hi1 = m1;
-        else lo1 = m1;
-      }
+      const double tol = fmax(ldexp(fmax(fabs(lo), fabs(hi)), -BIS_TOLBITS_EXT), floor_tol);
+      if (hi - lo <= tol) break;
+      const double mid = lo + 0.5 * (hi - lo);
+      if (sturm_count(tri, k, mid) > target) hi = mid;
+      else lo = mid;
     }
-    if (it == BIS_MAXIT) st |= ST_NOCONV;
-    lmin = lo0 + 0.5 * (hi0 - lo0);
-    lmax = lo1 + 0.5 * (hi1 - lo1);
+    noconv = it == BIS_MAXIT;
+    lam = lo + 0.5 * (hi - lo);
   }
-  a.lmin[s] = (float)lmin;
-  a.lmax[s] = (float)lmax;
-  a.delta[s] = (float)fmax(lmax - 1.0, 1.0 - lmin);
-  a.status[s] = st;
+  const double lmax = __shfl_xor(lam, 1, 64);
+  noconv |= __shfl_xor(noconv, 1, 64);
+  if (!upper) {
+    a.lmin[s] = (float)lam;
+    a.lmax[s] = (float)lmax;
+    a.delta[s] = (float)fmax(lmax - 1.0, 1.0 - lam);
+    a.status[s] = st | (noconv ? ST_NOCONV : 0u);
+  }
 }
 
 // every eigenvalue (uniform k): lane t of the block's targets bisects
@@ -2024,8 +1979,8 @@ __global__ __launch_bounds__(256) void det_kernel(BisectArgs a) {
 hipError_t launch_bisect(int mode, const BisectArgs &a, hipStream_t s) {
   if (a.nsub <= 0) return hipSuccess;
   if (mode == 0) {
-    const int64_t blocks = (a.nsub + BIS_BLOCK - 1) / BIS_BLOCK;
-    const size_t lds = (size_t)BIS_BLOCK * (a.tstride + 1) * sizeof(float2);
+    const int64_t blocks = (a.nsub + BIS_SUBS - 1) / BIS_SUBS;
+    const size_t lds = (size_t)BIS_SUBS * (a.tstride + 1) * sizeof(float2);
     if (lds > 65536) {
       hipError_t e = hipFuncSetAttribute((const void *)bisect_extreme_kernel,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
