@@ -992,10 +992,11 @@ __global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
 // in T (float, or double: FP64 reorthogonalisation coefficients) and applied
 // in an FP32 update.
 //   KC = 128, Q = 2: 256 lanes, L = 128 (two workgroups per CU)
-//   KC = 256, Q = 2: 512 lanes, L = 140
-//   KC = 256, Q = 4 (1024 lanes, four wavefronts per SIMD) compiles and
-//   agrees with Q = 2, but at 128 VGPRs per lane it spills inside the step
-//   loop and runs at half the speed; it is not offered as a variant.
+//   KC = 256, Q = 2: 512 lanes, L = 148 (the widest LDS share that fits: each
+//   column moved out of the VGPRs relieves the step loop's register pressure)
+//   Q = 4 (1024 lanes, four wavefronts per SIMD) agreed with Q = 2 at L = 140
+//   but spilled at 128 VGPRs per lane and ran at half the speed; its LDS
+//   share does not fit beside L = 148, so it is not instantiated.
 // ---------------------------------------------------------------------------
 template <int KC, int Q>
 struct HR {
@@ -1005,8 +1006,8 @@ struct HR {
   static constexpr int WPH = KC / 64;           // wavefronts per row part
   static constexpr int NW = Q * WPH;            // wavefronts
   static constexpr int LOGK = KC == 256 ? 8 : 7;
-  static constexpr int L = KC == 256 ? 140 : 128;  // basis columns in LDS
-  static constexpr int S = KC == 256 ? 140 : 132;  // LDS row stride: 16-byte rows, stride/4 odd
+  static constexpr int L = KC == 256 ? 148 : 128;  // basis columns in LDS
+  static constexpr int S = KC == 256 ? 148 : 132;  // LDS row stride: 16-byte rows, stride/4 odd
   static constexpr int R = (KC - L) / Q;        // register basis columns per lane
   static constexpr int RA = R > 0 ? R : 1;      // their array extent
   static_assert(S % 4 == 0 && (S / 4) % 2 == 1 && L % 4 == 0 && L <= S, "row layout");
@@ -1041,12 +1042,14 @@ struct HRShared {
     float part[Q][KC];  // Gram-Schmidt update partials of the row parts
     T rpart[Q][C::WPH][RP];  // register-column partials per part and wavefront
   };
-  T cp[C::NT];          // LDS-column dot partials: row group g, column c at g * ncl + c
+  union {
+    T cp[C::NT];        // LDS-column dot partials: row group g, column c at g * ncl + c
+    double redd[16];    // bounds reduction, after the recurrence
+  };
   double ra[C::NW];     // alpha partials (FP64) of all wavefronts
   double rb[C::WPH];    // beta^2 partials (FP64) of the wavefronts of part 0
   float rp[C::WPH];     // ||w||^2 partials before a Gram-Schmidt pass
   float red[16];
-  double redd[16];
   int ired[8];
 };
 template <int KC, int Q>
@@ -1054,7 +1057,6 @@ constexpr size_t hr_lds_bytes() {
   return (size_t)KC * HR<KC, Q>::S * sizeof(float) + sizeof(HRShared<KC, Q, double>);
 }
 static_assert(hr_lds_bytes<256, 2>() <= 163840, "LDS budget, 512 lanes");
-static_assert(hr_lds_bytes<256, 4>() <= 163840, "LDS budget, 1024 lanes");
 static_assert(2 * hr_lds_bytes<128, 2>() <= 163840, "LDS budget, 256 lanes: two workgroups per CU");
 
 template <typename T>
