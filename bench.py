@@ -27,6 +27,8 @@ Reported beside it, in the same JSON line:
     one FP32 library GEMM of the full product (set-up, outside the timing);
   * cpu_baseline: the FP64 oracle on the box's host cores over a bounded
     sample of the same subsets;
+  * c1, c2: the uniform-k configurations (k = 8 on a 256^2 G, k = 32 on a
+    4096^2 G over 2^20 subsets), tuned vs plain and accuracy vs dsyevd;
   * batching: the mixed k in {8..128} configuration (tuned binned persistent
     form eager / from a graph / after autotuning, vs the plain form);
   * precision: k = 128 and 256 with FP32 vs FP64 Gram-Schmidt coefficients,
@@ -146,6 +148,57 @@ def batching_leg(solver, torch, autotune=True):
     res["vs_plain"] = res["plain_ms"] / best
     del G
     torch.cuda.empty_cache()
+    return res
+
+
+def small_configs_leg(solver, torch, sample=4096):
+    """The two uniform-k configurations. C1: Phi 64 x 256 N(0,1/64) with unit
+    columns (seed 0xC1), 1024 subsets of k = 8 (stream 0xC1+1). C2: Phi
+    512 x 4096 N(0,1/512) with unit columns (seed 0xC2), 2^20 subsets of
+    k = 32 (stream 0xC2+1). Tuned and plain forms timed with HIP events on
+    torch's stream, accuracy of lambda_min / lambda_max against FP64 dsyevd on
+    every C1 subset and on a 4096-subset C2 sample (stream 0xC2+2 positions)."""
+    res = {}
+    for name, seed, m, n, k, nsub, reps in (("c1", 0xC1, 64, 256, 8, 1024, 50),
+                                            ("c2", 0xC2, 512, 4096, 32, 1 << 20, 5)):
+        G = solver.gram(solver.gen_phi("gauss", seed, m, n))
+        idx = solver.gen_subsets(seed + 1, nsub, k, n)
+        out = solver.solve(G, idx, k=k)
+        torch.cuda.synchronize()
+
+        def ev_ms(fn):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+
+        r = {"workload": f"Phi {m}x{n} N(0,1/{m}) unit columns, G {n}^2 FP32, {nsub} subsets of k={k}",
+             "subsets": nsub, "k": k}
+        r["tuned_ms"] = ev_ms(lambda: solver.solve(G, idx, k=k, out=out))
+        r["plain_ms"] = ev_ms(lambda: solver.solve(G, idx, k=k, form="plain", out=out))
+        r["solves_s_tuned"] = nsub / (r["tuned_ms"] / 1e3)
+        r["solves_s_plain"] = nsub / (r["plain_ms"] / 1e3)
+        r["vs_plain"] = r["plain_ms"] / r["tuned_ms"]
+        out = solver.solve(G, idx, k=k)
+        lmin, lmax, st = out[0].cpu().numpy(), out[1].cpu().numpy(), out[3].cpu().numpy()
+        rng = np.random.default_rng(seed + 2)
+        sel = np.sort(rng.choice(nsub, min(nsub, sample), replace=False))
+        A = solver.gather(G, idx.reshape(nsub, k)[torch.from_numpy(sel).to(idx.device)].reshape(-1),
+                          k, "dump", 0, 4).cpu().numpy().reshape(-1, k, k)
+        w = np.linalg.eigvalsh(A.astype(np.float64))
+        e = np.maximum(np.abs(lmin[sel] - w[:, 0]) / np.abs(w[:, 0]),
+                       np.abs(lmax[sel] - w[:, -1]) / np.abs(w[:, -1]))
+        r["accuracy"] = {"sample": int(len(sel)), "max_rel_err": float(e.max()),
+                         "median_rel_err": float(np.median(e)), "vs": "FP64 dsyevd",
+                         "nonzero_status": int(np.count_nonzero(st))}
+        res[name] = r
+        del G, idx, out
+        torch.cuda.empty_cache()
     return res
 
 
@@ -421,6 +474,7 @@ def main():
 
     legs = {}
     if not args.no_legs:
+        legs.update(small_configs_leg(solver, torch))
         legs["batching"] = batching_leg(solver, torch)
         legs["precision"] = precision_leg(solver, torch, dev)
 
