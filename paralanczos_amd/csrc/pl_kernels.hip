@@ -1025,13 +1025,14 @@ constexpr int k2_chunk() {
 // to the barrier after it, rpart from the reduce-scatter to the coefficient
 // sums, part from the update to the sum after the next barrier; lw (the
 // broadcast of w, and the index staging of the gather) until the column dots
-// end, lh from the coefficient sums to the update.
+// end, lh from the coefficient sums to the update; lvn from its store after
+// a Gram-Schmidt pass to the next matvec, cp from the column dots to the
+// coefficient sums.
 template <int KC, int Q, typename T>
 struct HRShared {
   using C = HR<KC, Q>;
   static constexpr int RP = C::R > 0 ? (C::R + k2_chunk<T>() - 1) / k2_chunk<T>() * k2_chunk<T>()
                                      : 1;  // register partial slots per (part, wavefront)
-  float lvn[KC];     // next Lanczos vector, unnormalised: the matvec input
   union {
     float lw[KC];    // w broadcast
     float lh[KC];    // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
@@ -1043,6 +1044,7 @@ struct HRShared {
     T rpart[Q][C::WPH][RP];  // register-column partials per part and wavefront
   };
   union {
+    float lvn[KC];      // next Lanczos vector, unnormalised: the matvec input
     T cp[C::NT];        // LDS-column dot partials: row group g, column c at g * ncl + c
     double redd[16];    // bounds reduction, after the recurrence
   };
@@ -1052,9 +1054,17 @@ struct HRShared {
   float red[16];
   int ired[8];
 };
+// The workgroup's LDS: the work buffers first, at addresses that the 16-bit
+// offset fields of LDS instructions reach (no address arithmetic for the
+// broadcasts of w and the coefficients), then V.
+template <int KC, int Q, typename T>
+struct HRLds {
+  HRShared<KC, Q, T> sh;
+  alignas(16) float VL[KC * HR<KC, Q>::S];
+};
 template <int KC, int Q>
 constexpr size_t hr_lds_bytes() {
-  return (size_t)KC * HR<KC, Q>::S * sizeof(float) + sizeof(HRShared<KC, Q, double>);
+  return sizeof(HRLds<KC, Q, double>);
 }
 static_assert(hr_lds_bytes<256, 2>() <= 163840, "LDS budget, 512 lanes");
 static_assert(2 * hr_lds_bytes<128, 2>() <= 163840, "LDS budget, 256 lanes: two workgroups per CU");
@@ -1207,6 +1217,20 @@ __device__ __forceinline__ int udiv_small(int a, int b) {
   return q;
 }
 
+// true on every lane when p holds on any lane of the workgroup (a barrier,
+// like __syncthreads_or, without the LDS block that one reserves):
+// per-wavefront flags in ired
+template <int KC, int Q, typename T>
+__device__ __forceinline__ bool hr_any(bool p, HRShared<KC, Q, T> &sh) {
+  const bool w = __ballot(p) != 0;
+  if ((threadIdx.x & 63) == 0) sh.ired[threadIdx.x >> 6] = w ? 1 : 0;
+  __syncthreads();
+  int any = 0;
+#pragma unroll
+  for (int u = 0; u < HR<KC, Q>::NW; ++u) any |= sh.ired[u];
+  return any != 0;
+}
+
 // a0..a3 += v[u] * w[r + u] over N rows (N a multiple of 4), w from the
 // broadcast copy in LDS (FP32 lw, or its FP64 copy lwT), accumulators by
 // row mod 4
@@ -1275,14 +1299,25 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   {
     const int g = udiv_small(t, ncl), c = t - g * ncl;
     if (g < ng) {
-      // RB rows per iteration (16 FP32, 8 FP64: register budget), then 4 at
-      // a time: the column entries of a batch are all loaded before the FMAs
-      // consume them, so each batch pays the LDS latency once
-      constexpr int RB = sizeof(T) == 8 ? 8 : 16;
+      // RB rows per iteration (16 FP32; 8 FP64, 4 at KC = 256: register
+      // budget), then 4 at a time: the column entries of a batch are all
+      // loaded before the FMAs consume them, so each batch pays the LDS
+      // latency once
+      constexpr int RB = sizeof(T) == 8 ? (KC == 256 ? 4 : 8) : 16;
       const int r0 = g * RR;
       const int rn = (min(RR, max(0, k - r0)) + 3) & ~3;
+      // ru = the rows of the last group, the fewest: a wave-uniform bound,
+      // so the main loop runs on a scalar counter; the tail is per lane
+      const int ru = __builtin_amdgcn_readfirstlane((min(RR, max(0, k - (ng - 1) * RR)) + 3) & ~3);
       T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
       int r = 0;
+      for (; r + RB <= ru; r += RB) {
+        const float *col = VL + (r0 + r) * C::S + c;
+        float v[RB];
+#pragma unroll
+        for (int u = 0; u < RB; ++u) v[u] = col[u * C::S];
+        hr_dot_rows<RB, KC, Q, T>(v, sh, r0 + r, a0, a1, a2, a3);
+      }
       for (; r + RB <= rn; r += RB) {
         const float *col = VL + (r0 + r) * C::S + c;
         float v[RB];
@@ -1347,7 +1382,7 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   {
     const int cs = ((ncl + 4 * Q - 1) / (4 * Q)) * 4;
     const int c0 = h * cs;
-    const int cn = max(0, min(cs, ncl - c0));
+    const int cn = __builtin_amdgcn_readfirstlane(max(0, min(cs, ncl - c0)));  // per part
     const float *row = VL + i * C::S + c0;
     const float *hl = sh.lh + c0;
     // 8 columns per iteration, then at most one group of 4; reads stay in
@@ -1421,7 +1456,7 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
     my = a.idx[off + i];
     if (my < 0 || (int64_t)my >= a.n) st = ST_BADINDEX;
   }
-  if (__syncthreads_or(st != 0)) st |= badk ? ST_BADK : ST_BADINDEX;
+  if (hr_any(st != 0, sh)) st |= badk ? ST_BADK : ST_BADINDEX;
   int *lidx = reinterpret_cast<int *>(sh.lw);
   if (h == 0) lidx[i] = my;
   __syncthreads();
@@ -1441,7 +1476,7 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
     g[c] = x;
     fin = fin && isfinite(x);
   }
-  if (__syncthreads_or(!fin)) st |= ST_NONFINITE;
+  if (hr_any(!fin, sh)) st |= ST_NONFINITE;
 
   if constexpr (MODE == MODE_DUMP) {
     if (i < k && st == 0) {
@@ -1577,7 +1612,7 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
         my_b = (i == j + 1) ? bj : my_b;
       }
       const bool finite = (i >= k) || (isfinite(my_a) && isfinite(my_b));
-      if (__syncthreads_or(!finite)) st |= ST_NONFINITE;
+      if (hr_any(!finite, sh)) st |= ST_NONFINITE;
       if (!(st & FATAL)) {
         float2 *tri = a.tri + s * (int64_t)a.tstride;
         sh.pw[0][i] = my_b;  // every part writes the same value
@@ -1604,8 +1639,7 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
 
 template <int KC, int Q, int MODE, typename T>
 __global__ __launch_bounds__(Q * KC) void solve_rowpart_kernel(SolveArgs a) {
-  __shared__ __attribute__((aligned(16))) float VL[KC * HR<KC, Q>::S];
-  __shared__ __attribute__((aligned(16))) HRShared<KC, Q, T> sh;
+  __shared__ __attribute__((aligned(16))) HRLds<KC, Q, T> lds;
   int64_t cnt = a.nslots;
   const int32_t *ord = a.order;
   if (a.bins) {
@@ -1613,7 +1647,7 @@ __global__ __launch_bounds__(Q * KC) void solve_rowpart_kernel(SolveArgs a) {
     ord = a.order + a.bins[BIN_START + a.bin];
   }
   for (int64_t slot = blockIdx.x; slot < cnt; slot += gridDim.x) {
-    hr_subset<KC, Q, MODE, T>(a, VL, sh, ord ? (int64_t)ord[slot] : slot);
+    hr_subset<KC, Q, MODE, T>(a, lds.VL, lds.sh, ord ? (int64_t)ord[slot] : slot);
     __syncthreads();
   }
 }
