@@ -20,7 +20,9 @@ Reported beside it, in the same JSON line:
   * accuracy: relative error of lambda_min / lambda_max of a 4096-subset
     sample of the served outputs against FP64 LAPACK dsyevd;
   * roofline of the dominant kernel (the k = 256 gather + Lanczos kernel,
-    FP64 Gram-Schmidt coefficients), timed with HIP events on its stream;
+    FP64 Gram-Schmidt coefficients), timed with HIP events on its stream,
+    against the FP32 vector peak, HBM, and (roofline_lds) the LDS bytes the
+    profiler counts for the same launch against the LDS peak;
   * the gather stage alone over the same subsets: algorithmic GB/s and the
     HBM fetch rate the profiler's counters give for it (profiles/);
   * gram: G = Phi^T Phi of the serving frame by pl_gram (matrix cores) against
@@ -56,6 +58,7 @@ KLIST = [16, 32, 64, 128, 256]
 SEED_PHI, SEED_SUB, SEED_NSUB, SEED_SAMPLE = 0xC5, 0xC5 + 1, 0xC5 + 3, 0xC5 + 2
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 256 CUs x 128 bytes per clock x 2.4 GHz = 78.6
 DOMINANT_K = 256
 
 
@@ -454,11 +457,21 @@ def main():
     F = n256 * flops_per_solve(DOMINANT_K)
     tflops = F / (dom_ms / 1e3) / 1e12
     gbs = B / (dom_ms / 1e3) / 1e9
-    traffic = None
+    traffic, lds = None, None
     prof = os.path.join(ROOT, "profiles", "solve_k256_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
-            traffic = json.load(f).get("traffic_bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("traffic_bytes_per_launch")
+        # LDS bytes the lanes request per launch (rocprof SQ_INSTS_LDS_*_BANDWIDTH
+        # of the same launch), over this run's launch time
+        if tj.get("lds_load_bytes_per_launch"):
+            lb = tj["lds_load_bytes_per_launch"] + tj.get("lds_store_bytes_per_launch", 0.0)
+            tbs = lb / (dom_ms / 1e3) / 1e12
+            lds = {"achieved": tbs, "peak": LDS_PEAK_TBS, "unit": "TB/s", "frac": tbs / LDS_PEAK_TBS,
+                   "bytes_per_launch": lb,
+                   "busy": tj["lds_idx_active_cycles_per_launch"]
+                   / (tj["grbm_gui_active_per_launch"] / 8 * 256)}
     # gather stage: algorithmic bytes (the k x k FP32 G_S + indices + one
     # output), and the HBM bytes the profiler counts for the same launch
     gB = n256 * (4 * DOMINANT_K * DOMINANT_K + 4 * DOMINANT_K + 4)
@@ -537,6 +550,7 @@ def main():
                      "launch_ms": dom_ms, "subsets_per_launch": n256},
         "roofline_hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS},
+        "roofline_lds": lds,
         "gather": gather,
         "gram": gram,
         **legs,
