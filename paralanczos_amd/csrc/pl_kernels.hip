@@ -1207,16 +1207,6 @@ __device__ __forceinline__ float hr_beta2(float w, HRShared<KC, Q, T> &sh) {
   return (float)tot;
 }
 
-// a / b for 0 <= a < 2^16, 1 <= b < 2^16: float reciprocal estimate, then
-// one correction step each way (the estimate is off by at most one)
-__device__ __forceinline__ int udiv_small(int a, int b) {
-  int q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
-  const int r = a - q * b;
-  q += (r >= b) ? 1 : 0;
-  q -= (r < 0) ? 1 : 0;
-  return q;
-}
-
 // true on every lane when p holds on any lane of the workgroup (a barrier,
 // like __syncthreads_or, without the LDS block that one reserves):
 // per-wavefront flags in ired
@@ -1229,6 +1219,51 @@ __device__ __forceinline__ bool hr_any(bool p, HRShared<KC, Q, T> &sh) {
 #pragma unroll
   for (int u = 0; u < HR<KC, Q>::NW; ++u) any |= sh.ired[u];
   return any != 0;
+}
+
+// Split of the column dots over the lanes, per number of LDS columns ncl
+// (1..L): ng = min(16, NT / ncl) row groups of RR rows (a multiple of 4,
+// ng RR >= KC), and the multiplier that turns lane index t < NT into its
+// row group t / ncl = umulhi(2 t, mul). Read from constant memory with a
+// wave-uniform index, so ng and RR arrive in scalar registers.
+struct DotSplit {
+  unsigned ng_rr;  // ng | RR << 8
+  unsigned mul;    // ceil(2^31 / ncl)
+};
+template <int KC, int Q>
+struct DotSplitTab {
+  DotSplit e[HR<KC, Q>::L + 1];
+};
+template <int KC, int Q>
+constexpr DotSplitTab<KC, Q> make_dot_split() {
+  DotSplitTab<KC, Q> tab{};
+  for (int ncl = 1; ncl <= HR<KC, Q>::L; ++ncl) {
+    const int ng = HR<KC, Q>::NT / ncl < 16 ? HR<KC, Q>::NT / ncl : 16;
+    const int rr = ((KC + ng - 1) / ng + 3) & ~3;
+    tab.e[ncl].ng_rr = (unsigned)ng | ((unsigned)rr << 8);
+    tab.e[ncl].mul = (unsigned)(((1ull << 31) + ncl - 1) / ncl);
+  }
+  return tab;
+}
+// every lane index maps to its exact quotient
+template <int KC, int Q>
+constexpr bool dot_split_exact() {
+  constexpr DotSplitTab<KC, Q> tab = make_dot_split<KC, Q>();
+  for (int ncl = 1; ncl <= HR<KC, Q>::L; ++ncl)
+    for (unsigned t = 0; t < (unsigned)HR<KC, Q>::NT; ++t)
+      if ((unsigned)(((unsigned long long)(2 * t) * tab.e[ncl].mul) >> 32) != t / (unsigned)ncl)
+        return false;
+  return true;
+}
+static_assert(dot_split_exact<256, 2>() && dot_split_exact<128, 2>(), "lane split quotient");
+__constant__ DotSplitTab<256, 2> kDotSplit256 = make_dot_split<256, 2>();
+__constant__ DotSplitTab<128, 2> kDotSplit128 = make_dot_split<128, 2>();
+template <int KC, int Q>
+__device__ __forceinline__ const DotSplit &dot_split(int ncl) {
+  if constexpr (KC == 256)
+    return kDotSplit256.e[ncl];
+  else
+    return kDotSplit128.e[ncl];
 }
 
 // a0..a3 += v[u] * w[r + u] over N rows (N a multiple of 4), w from the
@@ -1294,10 +1329,10 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   // ng RR >= KC), so nearly every lane works and the dependent chains stay
   // short; rows >= k are zero
   const int ncl = min(j + 1, C::L);
-  const int ng = min(16, udiv_small(C::NT, ncl));
-  const int RR = (udiv_small(KC + ng - 1, ng) + 3) & ~3;
+  const DotSplit sp = dot_split<KC, Q>(ncl);
+  const int ng = (int)(sp.ng_rr & 0xff), RR = (int)(sp.ng_rr >> 8);
   {
-    const int g = udiv_small(t, ncl), c = t - g * ncl;
+    const int g = (int)__umulhi(2u * (unsigned)t, sp.mul), c = t - g * ncl;
     if (g < ng) {
       // RB rows per iteration (16 FP32; 8 FP64, 4 at KC = 256: register
       // budget), then 4 at a time: the column entries of a batch are all
@@ -1606,8 +1641,14 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
         } else if constexpr (C::R > 0) {
           const int m = (c - C::L) / Q;
           const bool mine = ((c - C::L) % Q) == h;
+          // selects within the group of 8 slots holding m only (uniform branch)
 #pragma unroll
-          for (int r = 0; r < C::R; ++r) vr[r] = (mine && r == m) ? vi : vr[r];
+          for (int r8 = 0; r8 < C::R; r8 += 8) {
+            if ((m >> 3) == (r8 >> 3)) {
+#pragma unroll
+              for (int r = r8; r < r8 + 8 && r < C::R; ++r) vr[r] = (mine && r == m) ? vi : vr[r];
+            }
+          }
         }
         my_b = (i == j + 1) ? bj : my_b;
       }
