@@ -1879,11 +1879,12 @@ hipError_t launch_binning(const int64_t *offsets, int kuni, int64_t nsub, const 
 }
 
 // ---------------------------------------------------------------------------
-// Stage (c): FP64 Sturm-sequence bisection, one lane per eigenvalue sought.
-// A block of 64 lanes stages the T of its subsets into LDS once (row j of the
-// stage holds subset j's (alpha_i, beta_i), rows kstride+1 float2 apart so the
-// staging writes and the per-step reads hit distinct banks) and every lane
-// then bisects its own target from LDS.
+// Stage (c): FP64 Sturm-sequence bisection. A block of 64 lanes stages the T
+// of its subsets into LDS once (row j of the stage holds subset j's
+// (alpha_i, beta_i), rows kstride+1 float2 apart so the staging writes and
+// the per-step reads hit distinct banks); the extremes are then found by
+// multisection, 8 lanes per eigenvalue, and every eigenvalue of a subset
+// (the legacy path) by bisection, one lane each.
 // ---------------------------------------------------------------------------
 constexpr int BIS_BLOCK = 64;
 
@@ -1970,20 +1971,28 @@ __device__ __forceinline__ void stage_tridiag(const BisectArgs &a, int64_t s0, i
   __syncthreads();
 }
 
-// lambda_min / lambda_max: lanes 2j and 2j + 1 of a block bisect lambda_min
-// and lambda_max of the block's subset j; brackets [Gershgorin lo, min alpha]
-// and [max alpha, Gershgorin hi] hold since lambda_min(T) <= T_ii <=
-// lambda_max(T). Each extreme follows its own midpoint sequence and stops at
-// its own width, so the two lanes finish independently; the results meet in
-// lane 2j.
-constexpr int BIS_SUBS = BIS_BLOCK / 2;  // subsets per block
+// lambda_min / lambda_max by multisection: BIS_MS = 8 lanes per extreme,
+// BIS_SUBS = 4 subsets per 64-lane block (lanes 16 j .. 16 j + 7 seek
+// lambda_min of the block's subset j, lanes 16 j + 8 .. 16 j + 15 its
+// lambda_max). Each iteration the 8 lanes of an extreme evaluate the Sturm
+// count at the interior points lo + (r + 1) d, d = (hi - lo) / 9, r = 0..7,
+// and all move to the ninth of [lo, hi] that holds the target: the first
+// point whose count exceeds it (a ballot) is the new hi, the point before it
+// (or lo) the new lo, each the same double the evaluating lane formed.
+// Brackets [Gershgorin lo, min alpha] and [max alpha, Gershgorin hi] hold
+// since lambda_min(T) <= T_ii <= lambda_max(T), and the stopping width is
+// the bisection's; an iteration gains log2(9) = 3.2 bits instead of 1, so
+// the serial chain of Sturm sweeps a subset waits for is a third as long.
+constexpr int BIS_MS = 8;
+constexpr int BIS_SUBS = BIS_BLOCK / (2 * BIS_MS);  // subsets per block
 __global__ __launch_bounds__(BIS_BLOCK) void bisect_extreme_kernel(BisectArgs a) {
   extern __shared__ float2 stage[];
   const int64_t s0 = (int64_t)blockIdx.x * BIS_SUBS;
   const int cnt = (int)min<int64_t>(BIS_SUBS, a.nsub - s0);
   stage_tridiag(a, s0, cnt, stage);
-  const int j = threadIdx.x >> 1, upper = threadIdx.x & 1;
-  if (j >= cnt) return;  // both lanes of a pair leave together
+  const int grp = threadIdx.x / BIS_MS, r = threadIdx.x % BIS_MS;
+  const int j = grp >> 1, upper = grp & 1;
+  if (j >= cnt) return;  // both groups of a subset leave together
   const int64_t s = subset_at(a, s0 + j);
   const int k = subset_k(a, s);
   const uint32_t st = a.status[s];
@@ -2000,17 +2009,21 @@ __global__ __launch_bounds__(BIS_BLOCK) void bisect_extreme_kernel(BisectArgs a)
     int it = 0;
     for (; it < BIS_MAXIT; ++it) {
       const double tol = fmax(ldexp(fmax(fabs(lo), fabs(hi)), -BIS_TOLBITS_EXT), floor_tol);
-      if (hi - lo <= tol) break;
-      const double mid = lo + 0.5 * (hi - lo);
-      if (sturm_count(tri, k, mid) > target) hi = mid;
-      else lo = mid;
+      if (hi - lo <= tol) break;  // lo, hi are the same on the group's lanes
+      const double d = (hi - lo) * (1.0 / (BIS_MS + 1));
+      const bool above = sturm_count(tri, k, lo + (double)(r + 1) * d) > target;
+      const unsigned m = (unsigned)(__ballot(above) >> (grp * BIS_MS)) & ((1u << BIS_MS) - 1);
+      const int f = m ? __builtin_ctz(m) : BIS_MS;  // first point above the target
+      const double nlo = lo + (double)f * d;
+      hi = f < BIS_MS ? lo + (double)(f + 1) * d : hi;
+      lo = nlo;
     }
     noconv = it == BIS_MAXIT;
     lam = lo + 0.5 * (hi - lo);
   }
-  const double lmax = __shfl_xor(lam, 1, 64);
-  noconv |= __shfl_xor(noconv, 1, 64);
-  if (!upper) {
+  const double lmax = __shfl_xor(lam, BIS_MS, 64);
+  noconv |= __shfl_xor(noconv, BIS_MS, 64);
+  if (!upper && r == 0) {
     a.lmin[s] = (float)lam;
     a.lmax[s] = (float)lmax;
     a.delta[s] = (float)fmax(lmax - 1.0, 1.0 - lam);
