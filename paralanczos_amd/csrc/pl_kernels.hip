@@ -1971,21 +1971,26 @@ __device__ __forceinline__ void stage_tridiag(const BisectArgs &a, int64_t s0, i
   __syncthreads();
 }
 
-// lambda_min / lambda_max by multisection: BIS_MS = 8 lanes per extreme,
-// BIS_SUBS = 4 subsets per 64-lane block (lanes 16 j .. 16 j + 7 seek
-// lambda_min of the block's subset j, lanes 16 j + 8 .. 16 j + 15 its
-// lambda_max). Each iteration the 8 lanes of an extreme evaluate the Sturm
-// count at the interior points lo + (r + 1) d, d = (hi - lo) / 9, r = 0..7,
-// and all move to the ninth of [lo, hi] that holds the target: the first
-// point whose count exceeds it (a ballot) is the new hi, the point before it
-// (or lo) the new lo, each the same double the evaluating lane formed.
-// Brackets [Gershgorin lo, min alpha] and [max alpha, Gershgorin hi] hold
-// since lambda_min(T) <= T_ii <= lambda_max(T), and the stopping width is
-// the bisection's; an iteration gains log2(9) = 3.2 bits instead of 1, so
-// the serial chain of Sturm sweeps a subset waits for is a third as long.
-constexpr int BIS_MS = 8;
-constexpr int BIS_SUBS = BIS_BLOCK / (2 * BIS_MS);  // subsets per block
+// lambda_min / lambda_max by multisection: MS lanes per extreme (1 or 8),
+// BIS_BLOCK / (2 MS) subsets per 64-lane block (lanes 2 MS j .. 2 MS j +
+// MS - 1 seek lambda_min of the block's subset j, the next MS lanes its
+// lambda_max). Each iteration the MS lanes of an extreme evaluate the Sturm
+// count at the interior points lo + (r + 1) d, d = (hi - lo) / (MS + 1),
+// r = 0..MS-1, and all move to the part of [lo, hi] that holds the target:
+// the first point whose count exceeds it (a ballot) is the new hi, the point
+// before it (or lo) the new lo, each the same double the evaluating lane
+// formed. MS = 1 is plain bisection. Brackets [Gershgorin lo, min alpha] and
+// [max alpha, Gershgorin hi] hold since lambda_min(T) <= T_ii <=
+// lambda_max(T), and the stopping width is the same for both. With MS = 8
+// an iteration gains log2(9) = 3.2 bits instead of 1: the serial chain of
+// Sturm sweeps a subset waits for is a third as long, for 2.9x the sweeps,
+// which pays while the launch has few subsets (launch_bisect chooses).
+constexpr int BIS_MS_WIDE = 8;
+constexpr int BIS_WIDE_MAX_SUBSETS = 16384;  // multisection up to this many subsets
+template <int MS>
 __global__ __launch_bounds__(BIS_BLOCK) void bisect_extreme_kernel(BisectArgs a) {
+  constexpr int BIS_MS = MS;
+  constexpr int BIS_SUBS = BIS_BLOCK / (2 * MS);  // subsets per block
   extern __shared__ float2 stage[];
   const int64_t s0 = (int64_t)blockIdx.x * BIS_SUBS;
   const int cnt = (int)min<int64_t>(BIS_SUBS, a.nsub - s0);
@@ -2069,14 +2074,22 @@ __global__ __launch_bounds__(256) void det_kernel(BisectArgs a) {
 hipError_t launch_bisect(int mode, const BisectArgs &a, hipStream_t s) {
   if (a.nsub <= 0) return hipSuccess;
   if (mode == 0) {
-    const int64_t blocks = (a.nsub + BIS_SUBS - 1) / BIS_SUBS;
-    const size_t lds = (size_t)BIS_SUBS * (a.tstride + 1) * sizeof(float2);
+    const bool wide = a.nsub <= BIS_WIDE_MAX_SUBSETS;
+    const int subs = BIS_BLOCK / (2 * (wide ? BIS_MS_WIDE : 1));
+    const int64_t blocks = (a.nsub + subs - 1) / subs;
+    const size_t lds = (size_t)subs * (a.tstride + 1) * sizeof(float2);
+    const void *fn = wide ? (const void *)bisect_extreme_kernel<BIS_MS_WIDE>
+                          : (const void *)bisect_extreme_kernel<1>;
     if (lds > 65536) {
-      hipError_t e = hipFuncSetAttribute((const void *)bisect_extreme_kernel,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(bisect_extreme_kernel, dim3((unsigned)blocks), dim3(BIS_BLOCK), lds, s, a);
+    if (wide)
+      hipLaunchKernelGGL((bisect_extreme_kernel<BIS_MS_WIDE>), dim3((unsigned)blocks),
+                         dim3(BIS_BLOCK), lds, s, a);
+    else
+      hipLaunchKernelGGL((bisect_extreme_kernel<1>), dim3((unsigned)blocks), dim3(BIS_BLOCK),
+                         lds, s, a);
   } else {
     const int64_t threads = a.nsub * a.kuni;
     const int64_t blocks = (threads + BIS_BLOCK - 1) / BIS_BLOCK;
