@@ -1496,20 +1496,40 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
   if (h == 0) lidx[i] = my;
   __syncthreads();
 
-  // ---- (a) gather: lane (i, h) loads G_S[i][HW h + c] directly ----
+  // ---- (a) gather: lane (i, h) loads G_S[i][HW h + c] directly. The
+  // entries (i, c) and (c, i) are the same word of G; load v of every lane
+  // fetches column HW h + ((i mod HW) xor v), so the two lanes that need a
+  // word issue it with the same instruction and the second request meets
+  // the first line in the L2 instead of HBM. A butterfly of selects (bit b
+  // of i mod HW swaps the slots v and v xor 2^b) then puts column HW h + c
+  // into g[c]. ----
   float g[C::HW];
   bool fin = true;
+  {
+    const int im = i & (C::HW - 1);
 #pragma unroll
-  for (int c = 0; c < C::HW; ++c) {
-    const int col = C::HW * h + c;
-    float x = 0.f;
-    if (st == 0 && i < k && col < k) {
-      const int ic = lidx[col];
-      const int64_t lo = min(ic, my), hi = max(ic, my);
-      x = a.G[lo * a.ld + hi];
+    for (int v = 0; v < C::HW; ++v) {
+      const int col = C::HW * h + (im ^ v);
+      float x = 0.f;
+      if (st == 0 && i < k && col < k) {
+        const int ic = lidx[col];
+        const int64_t lo = min(ic, my), hi = max(ic, my);
+        x = a.G[lo * a.ld + hi];
+      }
+      g[v] = x;
+      fin = fin && isfinite(x);
     }
-    g[c] = x;
-    fin = fin && isfinite(x);
+#pragma unroll
+    for (int b = 1; b < C::HW; b <<= 1) {
+      const bool sw = (im & b) != 0;
+#pragma unroll
+      for (int v = 0; v < C::HW; ++v) {
+        if (v & b) continue;
+        const float lo = g[v], hi = g[v | b];
+        g[v] = sw ? hi : lo;
+        g[v | b] = sw ? lo : hi;
+      }
+    }
   }
   if (hr_any(!fin, sh)) st |= ST_NONFINITE;
 
