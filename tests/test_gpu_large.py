@@ -171,3 +171,19 @@ def test_rip_stats_per_size(gpu):
                          kmax=256).cpu().numpy()
     assert np.array_equal(u[:, [0, 5, 6, 7]], out[:, [0, 5, 6, 7]], equal_nan=True)
     np.testing.assert_allclose(u[1:, :], out[1:, :], rtol=1e-12, equal_nan=True)
+
+
+def test_multisection_agrees_with_bisection(gpu):
+    """Launches of up to 16384 subsets find the extremes by 9-way
+    multisection, larger ones by bisection. Both stop at the same width
+    (2^-27 relative) around the eigenvalue of the same T, so a subset's
+    extremes agree to that width and the FP32 rounding of the outputs."""
+    n = 4096
+    G = solver.gram(solver.gen_phi("gauss", 17, 512, n))
+    for k in (8, 40, 129, 256):
+        idx = solver.gen_subsets(70 + k, 16385, k, n)
+        big = _np(solver.solve(G, idx, k=k))  # 16385 subsets: bisection
+        small = _np(solver.solve(G, idx[:300 * k].contiguous(), k=k))  # multisection
+        assert not small[3].any() and not big[3][:300].any()
+        for a, b in ((small[0], big[0][:300]), (small[1], big[1][:300])):
+            assert rel(a, b).max() <= 2.0 ** -22, k
