@@ -94,6 +94,15 @@ Variant plain_variant(int k) {
   return Variant{k <= 64 ? 64 : k <= 128 ? 128 : 512, 4, false};
 }
 
+Variant plain_wave_variant(int k) {
+  // one 64-lane wavefront per subset at every k
+  return k <= 64 ? Variant{64, 4, false} : Variant{64, 0, false, true};
+}
+
+bool is_plain(int form) {
+  return (form & kFormMask) == PL_FORM_PLAIN || (form & kFormMask) == PL_FORM_PLAIN_WAVE;
+}
+
 // Bins of a mixed-size batch. Tuned: maximal runs of consecutive k sharing a
 // variant (the bin edges follow the tuning table). Plain: one bin per exact k.
 // Bin 0 also takes every size outside [1, kmax]; its launch flags them BADK.
@@ -105,14 +114,16 @@ struct BinPlan {
 };
 
 Variant job_variant(int form, int k) {
-  Variant v = (form & kFormMask) == PL_FORM_PLAIN ? plain_variant(k) : tuned_variant(k);
+  Variant v = (form & kFormMask) == PL_FORM_PLAIN        ? plain_variant(k)
+              : (form & kFormMask) == PL_FORM_PLAIN_WAVE ? plain_wave_variant(k)
+                                                         : tuned_variant(k);
   v.f64 = use_f64(form, k, v.lanes);
   return v;
 }
 
 void make_bins(int form, int kmax, BinPlan &bp) {
   bp.nbins = 0;
-  if ((form & kFormMask) == PL_FORM_PLAIN) {
+  if (is_plain(form)) {
     bp.var[0] = job_variant(form, 1);
     bp.kbin[0] = 0;
     for (int k = 1; k <= kmax; ++k) {
@@ -216,7 +227,23 @@ pl_status run_bisect(const SolveJob &j, const pl::WorkspaceParts &w, const int32
 // Whether a job runs without host synchronisation (and so can be captured
 // into a graph): all but the plain form of a mixed-size batch.
 bool job_is_async(const SolveJob &j) {
+  if (j.form != -1 && (j.form & kFormMask) == PL_FORM_PLAIN_WAVE) return false;
   return !(j.offsets && j.form != -1 && (j.form & kFormMask) == PL_FORM_PLAIN);
+}
+
+// one launch of a variant over the job's slots; the one-wavefront variant
+// with a scratch allocated on the stream for the launch
+hipError_t launch_variant(const Variant &v, const pl::SolveArgs &a, hipStream_t s,
+                          int64_t max_slots) {
+  if (!v.wave) return pl::launch_solve(v.lanes, v.pad, pl::MODE_TRIDIAG, a, s, max_slots, v.f64);
+  const int64_t want = a.bins ? max_slots : a.nslots;
+  const int64_t slots = std::max<int64_t>(1, std::min<int64_t>(want, pl::kWaveSlots));
+  void *scratch = nullptr;
+  hipError_t e = hipMallocAsync(&scratch, pl::wave_scratch_bytes(slots), s);
+  if (e != hipSuccess) return e;
+  e = pl::launch_wave(a, (float *)scratch, slots, s);
+  const hipError_t f = hipFreeAsync(scratch, s);
+  return e != hipSuccess ? e : f;
 }
 
 // The stream-ordered work of one solve on a workspace of
@@ -236,7 +263,7 @@ pl_status run_job(const SolveJob &j, void *ws, hipStream_t s, int stages = 3) {
     // uniform size, or an explicit variant over a mixed batch: one launch
     if (stages & kStageTridiag) {
       const Variant v = j.form == -1 ? j.explicit_v : job_variant(j.form, j.k);
-      e = pl::launch_solve(v.lanes, v.pad, pl::MODE_TRIDIAG, a, s, 0, v.f64);
+      e = launch_variant(v, a, s, 0);
       if (e != hipSuccess) return hip_fail(e, "solve launch");
     }
     return (stages & kStageBisect) ? run_bisect(j, w, nullptr, s) : PL_OK;
@@ -266,8 +293,7 @@ pl_status run_job(const SolveJob &j, void *ws, hipStream_t s, int stages = 3) {
       if (cnt == 0) continue;
       a.order = w.order + tab[pl::BIN_START + b];
       a.nslots = cnt;
-      e = pl::launch_solve(bp.var[b].lanes, bp.var[b].pad, pl::MODE_TRIDIAG, a, s, 0,
-                           bp.var[b].f64);
+      e = launch_variant(bp.var[b], a, s, 0);
       if (e != hipSuccess) return hip_fail(e, "per-k solve launch");
     }
   }
@@ -286,9 +312,11 @@ pl_status check_job(const SolveJob &j) {
     if (!pl::solve_variant_exists(j.explicit_v.lanes, j.explicit_v.pad))
       return fail(PL_ERR_ARG, "no variant with %d lanes and pad %d", j.explicit_v.lanes,
                   j.explicit_v.pad);
-  } else if (((j.form & kFormMask) != PL_FORM_TUNED && (j.form & kFormMask) != PL_FORM_PLAIN) ||
+  } else if (((j.form & kFormMask) != PL_FORM_TUNED && !is_plain(j.form)) ||
              (j.form & ~(kFormMask | kPrecMask)) || (j.form & kPrecMask) == kPrecMask) {
     return fail(PL_ERR_ARG, "unknown form %d", j.form);
+  } else if ((j.form & kFormMask) == PL_FORM_PLAIN_WAVE && (j.form & kPrecMask)) {
+    return fail(PL_ERR_ARG, "the one-wavefront form is FP32 only (no precision flags)");
   }
   return PL_OK;
 }

@@ -98,6 +98,13 @@ inline WorkspaceParts split_workspace(void *p, int64_t nsub, int kstride) {
 hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStream_t s,
                         int64_t max_slots = 0, bool f64 = false);
 bool solve_variant_exists(int group, int pad);
+// one 64-lane wavefront per subset at any k <= 256, G_S and the basis in
+// `scratch` (wave_scratch_bytes(slots)); `slots` resident wavefronts
+constexpr int64_t kWaveSlots = 2048;
+inline size_t wave_scratch_bytes(int64_t slots) {
+  return (size_t)slots * 2 * 256 * 256 * sizeof(float);
+}
+hipError_t launch_wave(const SolveArgs &a, float *scratch, int64_t slots, hipStream_t s);
 // largest k a variant with this many lanes per subset takes
 int variant_capacity(int group);
 
@@ -125,7 +132,8 @@ hipError_t launch_gen_phi_dct(const int32_t *rows_dev, int m, int n, float *phi,
 // ---- host-side solve jobs (pl_capi.hip), shared with the serving engine ----
 struct Variant {
   int lanes, pad;
-  bool f64 = false;  // FP64 Gram-Schmidt coefficients
+  bool f64 = false;   // FP64 Gram-Schmidt coefficients
+  bool wave = false;  // one 64-lane wavefront beyond k = 64 (global scratch)
 };
 
 // everything one solve call needs, bound to device buffers

@@ -1805,6 +1805,233 @@ hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStr
 }
 
 // ---------------------------------------------------------------------------
+// PL_FORM_PLAIN_WAVE: one 64-lane wavefront per subset at every k <= 256.
+// Beyond k = 64, G_S and the Lanczos basis do not fit a wavefront's share of
+// the registers and LDS, so they live in a global scratch slot of the
+// resident wavefront (2 x 256 x 256 FP32): column c of G_S (equal to row c)
+// and basis vector c are each a run of 256 floats, read coalesced (lane l
+// takes entries l + 64 m, m < 4) and streamed every step. FP32 matvec and
+// Gram-Schmidt (coefficients by wavefront sums); alpha and beta^2 as FP64
+// sums of exact products; one classical Gram-Schmidt pass, a second one by
+// the DGKS test; the breakdown rule of the other kernels (restart with the
+// e_i of largest residual, orthogonalised twice). T and the bounds go to
+// the workspace like every stage-1 kernel's.
+// ---------------------------------------------------------------------------
+constexpr int WV_K = 256, WV_M = WV_K / 64;
+
+// w -= V[:, 0..j] (V[:, 0..j]^T w), one pass; coefficients through lh
+__device__ __forceinline__ void wv_cgs(float (&w)[WV_M], int j, const float *Vs, float *lh) {
+  const int l = threadIdx.x;
+  for (int c = 0; c <= j; ++c) {
+    const float *vc = Vs + (size_t)c * WV_K + l;
+    float p = 0.f;
+#pragma unroll
+    for (int m = 0; m < WV_M; ++m) p = __builtin_fmaf(vc[64 * m], w[m], p);
+    p = g_sum<64>(p);
+    if (l == 0) lh[c] = p;
+  }
+  __syncthreads();
+  for (int c = 0; c <= j; ++c) {
+    const float *vc = Vs + (size_t)c * WV_K + l;
+    const float hc = lh[c];
+#pragma unroll
+    for (int m = 0; m < WV_M; ++m) w[m] = __builtin_fmaf(-vc[64 * m], hc, w[m]);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double wv_sum_sq(const float (&w)[WV_M]) {
+  double x = 0.0;
+#pragma unroll
+  for (int m = 0; m < WV_M; ++m) x += (double)w[m] * (double)w[m];
+  return wave_sum_f64(x);
+}
+
+__global__ __launch_bounds__(64) void solve_wave_kernel(SolveArgs a, float *scratch) {
+  __shared__ float lv[WV_K];   // matvec input
+  __shared__ float lh[WV_K];   // Gram-Schmidt coefficients
+  __shared__ int lidx[WV_K];   // the subset's indices
+  __shared__ float la[WV_K], lb[WV_K];
+  float *Gs = scratch + (size_t)blockIdx.x * 2 * WV_K * WV_K;
+  float *Vs = Gs + (size_t)WV_K * WV_K;
+  const int l = threadIdx.x;
+  int64_t cnt = a.nslots;
+  const int32_t *ord = a.order;
+  if (a.bins) {
+    cnt = a.bins[BIN_COUNT + a.bin];
+    ord = a.order + a.bins[BIN_START + a.bin];
+  }
+  for (int64_t slot = blockIdx.x; slot < cnt; slot += gridDim.x) {
+    const int64_t s = ord ? (int64_t)ord[slot] : slot;
+    int k;
+    int64_t off;
+    if (a.offsets) {
+      off = a.offsets[s];
+      k = (int)(a.offsets[s + 1] - off);
+    } else {
+      k = a.kuni;
+      off = s * (int64_t)k;
+    }
+    const bool badk = k < 1 || k > WV_K || k > a.kcap;
+    uint32_t st = badk ? ST_BADK : 0u;
+    bool bad = false;
+#pragma unroll
+    for (int m = 0; m < WV_M; ++m) {
+      const int r = l + 64 * m;
+      int x = 0;
+      if (!badk && r < k) {
+        x = a.idx[off + r];
+        bad = bad || x < 0 || (int64_t)x >= a.n;
+      }
+      lidx[r] = x;
+    }
+    if (__ballot(bad) != 0) st |= ST_BADINDEX;
+    __syncthreads();
+    // ---- (a) gather: G_S column c (= row c) into scratch row c, zeros in
+    // rows k..255 (the matvec reads whole rows) ----
+    bool fin = true;
+    if (!st) {
+      for (int c = 0; c < k; ++c) {
+        const int ic = lidx[c];
+#pragma unroll
+        for (int m = 0; m < WV_M; ++m) {
+          const int r = l + 64 * m;
+          float x = 0.f;
+          if (r < k) {
+            const int ir = lidx[r];
+            const int64_t lo = min(ic, ir), hi = max(ic, ir);
+            x = a.G[lo * a.ld + hi];
+            fin = fin && isfinite(x);
+          }
+          Gs[(size_t)c * WV_K + r] = x;
+        }
+      }
+    }
+    if (__ballot(!fin) != 0) st |= ST_NONFINITE;
+    __syncthreads();
+    constexpr uint32_t FATAL = ST_BADK | ST_BADINDEX | ST_NONFINITE;
+    if (!(st & FATAL)) {
+      // ---- (b) Lanczos, full reorthogonalisation ----
+      float v[WV_M], vp[WV_M], w[WV_M];
+      const float sc0 = 1.0f / sqrtf((float)k);
+#pragma unroll
+      for (int m = 0; m < WV_M; ++m) {
+        const int r = l + 64 * m;
+        v[m] = r < k ? sc0 : 0.f;
+        vp[m] = 0.f;
+        Vs[r] = v[m];
+        lv[r] = v[m];
+      }
+      __syncthreads();
+      float bj = 0.f, amax = 0.f, bmax = 0.f;
+      for (int j = 0; j < k; ++j) {
+        float y[WV_M] = {0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < k; ++c) {
+          const float *gc = Gs + (size_t)c * WV_K + l;
+          const float vc = lv[c];
+#pragma unroll
+          for (int m = 0; m < WV_M; ++m) y[m] = __builtin_fmaf(gc[64 * m], vc, y[m]);
+        }
+        double pa = 0.0;
+#pragma unroll
+        for (int m = 0; m < WV_M; ++m) pa += (double)v[m] * (double)y[m];
+        const float al = (float)wave_sum_f64(pa);
+        if (l == 0) la[j] = al;
+        amax = fmaxf(amax, fabsf(al));
+        if (j == k - 1) break;
+#pragma unroll
+        for (int m = 0; m < WV_M; ++m) w[m] = __builtin_fmaf(-al, v[m], __builtin_fmaf(-bj, vp[m], y[m]));
+        const float pre = (float)wv_sum_sq(w);
+        wv_cgs(w, j, Vs, lh);
+        float b2 = (float)wv_sum_sq(w);
+        if (b2 < 0.5f * pre) {
+          wv_cgs(w, j, Vs, lh);
+          b2 = (float)wv_sum_sq(w);
+        }
+        float b = sqrtf(b2);
+        float scale;
+        if (b > TAU * (amax + bmax)) {
+          bj = b;
+          bmax = fmaxf(bmax, b);
+          scale = 1.0f / b;
+        } else {
+          // restart: e_i for the first i of largest residual 1 - ||V[i][:]||^2
+          st |= ST_BREAKDOWN;
+          bj = 0.f;
+          float best = -1.f;
+          int bi = 0;
+#pragma unroll
+          for (int m = 0; m < WV_M; ++m) {
+            const int r = l + 64 * m;
+            float rs = 0.f;
+            for (int c = 0; c <= j; ++c) {
+              const float x = Vs[(size_t)c * WV_K + r];
+              rs = __builtin_fmaf(x, x, rs);
+            }
+            const float res = r < k ? 1.f - rs : -1.f;
+            if (res > best) {
+              best = res;
+              bi = r;
+            }
+          }
+          bi = restart_lane<64>(best, bi, WV_K);
+#pragma unroll
+          for (int m = 0; m < WV_M; ++m) w[m] = (l + 64 * m == bi) ? 1.f : 0.f;
+          wv_cgs(w, j, Vs, lh);
+          wv_cgs(w, j, Vs, lh);
+          scale = 1.0f / sqrtf((float)wv_sum_sq(w));
+        }
+        if (l == 0) lb[j + 1] = bj;
+#pragma unroll
+        for (int m = 0; m < WV_M; ++m) {
+          const int r = l + 64 * m;
+          vp[m] = v[m];
+          v[m] = w[m] * scale;
+          Vs[(size_t)(j + 1) * WV_K + r] = v[m];
+          lv[r] = v[m];
+        }
+        __syncthreads();
+      }
+      if (l == 0) lb[0] = 0.f;
+      __syncthreads();
+      // ---- T, bounds ----
+      bool finite = true;
+      double glo = INFINITY, ghi = -INFINITY, amin = INFINITY, amx = -INFINITY;
+      float2 *tri = a.tri + s * (int64_t)a.tstride;
+#pragma unroll
+      for (int m = 0; m < WV_M; ++m) {
+        const int r = l + 64 * m;
+        if (r < k) {
+          const float ar = la[r], br = lb[r], bn = r + 1 < k ? lb[r + 1] : 0.f;
+          finite = finite && isfinite(ar) && isfinite(br);
+          tri[r] = make_float2(ar, br);
+          const double rad = (double)fabsf(br) + (double)fabsf(bn);
+          glo = fmin(glo, (double)ar - rad);
+          ghi = fmax(ghi, (double)ar + rad);
+          amin = fmin(amin, (double)ar);
+          amx = fmax(amx, (double)ar);
+        }
+      }
+      if (__ballot(!finite) != 0) st |= ST_NONFINITE;
+      glo = g_min<64>(glo);
+      ghi = g_max<64>(ghi);
+      amin = g_min<64>(amin);
+      amx = g_max<64>(amx);
+      if (l == 0 && !(st & ST_NONFINITE)) a.bounds[s] = make_double4(glo, ghi, amin, amx);
+    }
+    if (l == 0) a.status[s] = st;
+    __syncthreads();
+  }
+}
+
+hipError_t launch_wave(const SolveArgs &a, float *scratch, int64_t slots, hipStream_t s) {
+  int64_t blocks = a.bins ? slots : std::min<int64_t>(slots, a.nslots);
+  if (blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(solve_wave_kernel, dim3((unsigned)blocks), dim3(64), 0, s, a, scratch);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Binning of subsets by size: count per bin (LDS histogram per block, one
 // global atomic per bin and block), exclusive scan, scatter of subset ids into
 // the order array (LDS atomics give each subset its place within the block's

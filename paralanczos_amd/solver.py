@@ -13,7 +13,7 @@ import torch
 
 from ._lib import ParaLanczosError, check, load
 
-FORMS = {"tuned": 0, "plain": 1}
+FORMS = {"tuned": 0, "plain": 1, "plain_wave": 2}
 REORTH = {None: 0, "auto": 0, "f32": 0x100, "f64": 0x200}
 PHI_KINDS = {"gauss": 0, "bernoulli": 1, "dct": 2}
 
