@@ -14,9 +14,13 @@ them (closed loop); the value is subsets solved per second over K steps.
 Reported beside it, in the same JSON line:
   * latency: p50/p99 of the closed-loop steps, and of Poisson arrivals at
     fractions of the measured saturation rate;
-  * plain: the same subsets solved as one CSR batch by the plain form (one
-    launch per distinct k, FP32 throughout), and the tuned batch without the
-    serving engine;
+  * plain: the same subsets solved as one CSR batch by the plain form, one
+    launch per distinct k, FP32 throughout, one 64-lane wavefront per subset
+    (PL_FORM_PLAIN_WAVE: beyond k = 64 the wavefront streams G_S and its
+    basis from a global scratch); vs_plain is the served rate over it.
+    plain_multiwave / vs_plain_multiwave: the same against the plain form
+    that runs k > 64 on the multi-wavefront kernels (PL_FORM_PLAIN). And
+    the tuned batch without the serving engine;
   * accuracy: relative error of lambda_min / lambda_max of a 4096-subset
     sample of the served outputs against FP64 LAPACK dsyevd;
   * roofline of the dominant kernel (the k = 256 gather + Lanczos kernel,
@@ -112,7 +116,9 @@ def batching_leg(solver, torch, autotune=True):
     """Mixed sizes k in {8, 16, ..., 128}: Phi 1024 x 8192 N(0,1/1024) with
     unit columns (seed 0xC3), 2^20 subsets (sizes and index sets from streams
     0xC3+1). Tuned binned persistent form (eager, and replayed from a graph),
-    the plain one-launch-per-k form, and the tuned form after autotuning."""
+    the plain one-launch-per-k forms (one wavefront per subset; and with the
+    multi-wavefront kernels beyond k = 64), and the tuned form after
+    autotuning."""
     n, nsub = 8192, 1 << 20
     G = solver.gram(solver.gen_phi("gauss", 0xC3, 1024, n))
     ks = solver.gen_ks(0xC3 + 1, nsub, list(range(8, 129, 8)))
@@ -132,7 +138,10 @@ def batching_leg(solver, torch, autotune=True):
     out = solver.solve(G, idx, k=128, offsets=off)
     res = {"workload": "Phi 1024x8192 N(0,1/1024) unit columns, G 8192^2 FP32, 2^20 subsets, "
                        "k uniform in {8,16,...,128}", "subsets": nsub}
-    res["plain_ms"] = ms(lambda: solver.solve(G, idx, k=128, offsets=off, form="plain", out=out))
+    res["plain_ms"] = ms(lambda: solver.solve(G, idx, k=128, offsets=off, form="plain_wave",
+                                              out=out), reps=1)
+    res["plain_multiwave_ms"] = ms(lambda: solver.solve(G, idx, k=128, offsets=off, form="plain",
+                                                        out=out))
     res["tuned_ms"] = ms(lambda: solver.solve(G, idx, k=128, offsets=off, out=out))
     plan = solver.Plan(G, idx, k=128, offsets=off)
     res["tuned_graph_ms"] = ms(plan.execute)
@@ -149,6 +158,7 @@ def batching_leg(solver, torch, autotune=True):
     res["solves_s_tuned_best"] = nsub / (best / 1e3)
     res["solves_s_plain"] = nsub / (res["plain_ms"] / 1e3)
     res["vs_plain"] = res["plain_ms"] / best
+    res["vs_plain_multiwave"] = res["plain_multiwave_ms"] / best
     del G
     torch.cuda.empty_cache()
     return res
@@ -158,7 +168,8 @@ def small_configs_leg(solver, torch, sample=4096):
     """The two uniform-k configurations. C1: Phi 64 x 256 N(0,1/64) with unit
     columns (seed 0xC1), 1024 subsets of k = 8 (stream 0xC1+1). C2: Phi
     512 x 4096 N(0,1/512) with unit columns (seed 0xC2), 2^20 subsets of
-    k = 32 (stream 0xC2+1). Tuned and plain forms timed with HIP events on
+    k = 32 (stream 0xC2+1). Tuned and plain forms (at k <= 64 both plain forms
+    are the same one-wavefront kernel) timed with HIP events on
     torch's stream, accuracy of lambda_min / lambda_max against FP64 dsyevd on
     every C1 subset and on a 4096-subset C2 sample (stream 0xC2+2 positions)."""
     res = {}
@@ -419,7 +430,7 @@ def main():
     off_all = solver.offsets_from_ks(offs)
     idx_all = torch.cat([q["idx"] for q in reqs])
     batch = {}
-    for form in ("plain", "tuned"):
+    for form in ("plain_wave", "plain", "tuned"):
         o = solver.solve(G, idx_all, k=256, offsets=off_all, form=form)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -538,9 +549,11 @@ def main():
         "p50_ms": closed["p50_s"] * 1e3,
         "p99_ms": closed["p99_s"] * 1e3,
         "latency_poisson": latency,
-        "plain": batch["plain"],
+        "plain": batch["plain_wave"],
+        "plain_multiwave": batch["plain"],
         "tuned_batch": batch["tuned"],
-        "vs_plain": value / batch["plain"]["value"],
+        "vs_plain": value / batch["plain_wave"]["value"],
+        "vs_plain_multiwave": value / batch["plain"]["value"],
         "accuracy": {"sample": int(len(errs)), "max_rel_err": float(errs.max()),
                      "median_rel_err": float(np.median(errs)), "vs": "FP64 dsyevd",
                      "nonzero_status": int(np.count_nonzero(status))},
