@@ -151,3 +151,16 @@ def test_matlab_wrappers_match_python_restatement():
         assert "pl_multi_subsvd_call" in text
     call = open(os.path.join(mdir, "pl_multi_subsvd_call.m")).read()
     assert "calllib('libparalanczos', 'pl_multi_subsvd'" in call
+
+
+def test_python_form_constants_match_the_header():
+    """solver.FORMS / REORTH / layout values are the header's #defines."""
+    from paralanczos_amd import solver
+    text = open(os.path.join(ROOT, "include", "paralanczos.h")).read()
+    defs = {m.group(1): int(m.group(2), 0)
+            for m in re.finditer(r"#define\s+(PL_\w+)\s+(0x[0-9a-fA-F]+|\d+)", text)}
+    assert solver.FORMS == {"tuned": defs["PL_FORM_TUNED"], "plain": defs["PL_FORM_PLAIN"],
+                            "plain_wave": defs["PL_FORM_PLAIN_WAVE"]}
+    assert solver.REORTH["f32"] == defs["PL_FORM_REORTH_F32"]
+    assert solver.REORTH["f64"] == defs["PL_FORM_REORTH_F64"]
+    assert solver.LAYOUTS["row"] == defs["PL_LAYOUT_ROW"] and solver.LAYOUTS["col"] == defs["PL_LAYOUT_COL"]
