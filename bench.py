@@ -40,11 +40,12 @@ Reported beside it, in the same JSON line:
   * precision: k = 128 and 256 with FP32 vs FP64 Gram-Schmidt coefficients,
     throughput and error vs FP64 dsyevd.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--requests 64] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--requests 64] [--no-cpu]
 """
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import math
 import os
@@ -288,7 +289,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--requests", type=int, default=64, help="requests per step")
     ap.add_argument("--rates", default="0.5,0.8",
                     help="Poisson arrival rates as fractions of the saturation rate")
@@ -406,6 +407,8 @@ def main():
     torch.cuda.synchronize()
     srv.stats(reset=True)
     step_s = []
+    gc.collect()
+    gc.disable()  # no collector pauses in the host loop that submits and waits
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
@@ -413,6 +416,7 @@ def main():
         step_s.append(time.perf_counter() - ts)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    gc.enable()
     closed = srv.stats()
     ms_per_step = wall * 1e3 / args.steps
     value = tot * args.steps / wall
