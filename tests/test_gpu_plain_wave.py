@@ -78,3 +78,23 @@ def test_plain_wave_flags(gpu):
     rc = L.pl_solve(P(G), n, n, P(idx), None, 100, 4, *[P(x) for x in o], 2 | 0x200,
                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
     assert rc != 0
+
+
+def test_plain_wave_rank_deficient(gpu):
+    """m = 96 < k: G_S has rank 96, the Krylov space of the start vector is
+    exhausted early and the recurrence restarts (BREAKDOWN) or continues on
+    the reorthogonalised remainder; the extremes still agree with LAPACK,
+    lambda_min = 0 to 1e-5 of lambda_max."""
+    n = 600
+    G = solver.gram(solver.gen_phi("gauss", 5, 96, n))
+    Gh = G.cpu().numpy()
+    for k in (100, 200):
+        idx = solver.gen_subsets(60 + k, 8, k, n)
+        lo, hi, de, st = _np(solver.solve(G, idx, k=k, form="plain_wave"))
+        assert np.all((st & ~1) == 0)
+        ih = idx.cpu().numpy()
+        for s in range(8):
+            A, _ = oracle.gather(Gh, ih[s * k:(s + 1) * k])
+            w = np.linalg.eigvalsh(A)
+            assert abs(lo[s] - w[0]) <= 1e-5 * w[-1], (k, s)
+            assert abs(hi[s] - w[-1]) <= 1e-5 * w[-1], (k, s)
