@@ -32,7 +32,8 @@ Reported beside it, in the same JSON line:
   * gram: G = Phi^T Phi of the serving frame by pl_gram (matrix cores) against
     one FP32 library GEMM of the full product (set-up, outside the timing);
   * cpu_baseline: the FP64 oracle on the box's host cores over a bounded
-    sample of the same subsets;
+    sample of the same subsets, and (c1) over the whole k = 8 configuration
+    on one thread and on all cores;
   * c1, c2: the uniform-k configurations (k = 8 on a 256^2 G, k = 32 on a
     4096^2 G over 2^20 subsets), tuned vs plain and accuracy vs dsyevd;
   * batching: the mixed k in {8..128} configuration (tuned binned persistent
@@ -529,6 +530,20 @@ def main():
                          f"oracle (full reorth + Sturm bisection), OpenMP over subsets, "
                          f"{spent:.1f} s"}
         del Gh
+        # the small uniform configuration runs whole on the host: Phi 64 x 256
+        # (seed 0xC1), 1024 subsets of k = 8 (stream 0xC1+1), one thread and
+        # all cores
+        G1 = solver.gram(solver.gen_phi("gauss", 0xC1, 64, 256)).cpu().numpy()
+        i1 = solver.gen_subsets(0xC1 + 1, 1024, 8, 256).cpu().numpy()
+        c1 = {}
+        for th in (1, cores):
+            oracle.solve_batch(G1, i1, k=8, nthreads=th)  # warm
+            reps, t2 = 0, time.perf_counter()
+            while time.perf_counter() - t2 < 0.5:
+                oracle.solve_batch(G1, i1, k=8, nthreads=th)
+                reps += 1
+            c1[f"solves_s_{th}_threads"] = 1024 * reps / (time.perf_counter() - t2)
+        cpu["c1"] = dict(c1, workload="Phi 64x256, 1024 subsets of k=8, FP64 oracle")
 
     line = {
         "metric": METRIC,
