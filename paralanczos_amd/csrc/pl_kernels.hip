@@ -1819,16 +1819,25 @@ hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStr
 // ---------------------------------------------------------------------------
 constexpr int WV_K = 256, WV_M = WV_K / 64;
 
-// w -= V[:, 0..j] (V[:, 0..j]^T w), one pass; coefficients through lh
+// w -= V[:, 0..j] (V[:, 0..j]^T w), one pass; coefficients through lh. The
+// lanes' partial dots of 16 columns at a time are summed by one wavefront
+// reduce-scatter (4 lanes end with each column's sum).
 __device__ __forceinline__ void wv_cgs(float (&w)[WV_M], int j, const float *Vs, float *lh) {
   const int l = threadIdx.x;
-  for (int c = 0; c <= j; ++c) {
-    const float *vc = Vs + (size_t)c * WV_K + l;
-    float p = 0.f;
+  for (int c0 = 0; c0 <= j; c0 += 16) {
+    float x[16];
 #pragma unroll
-    for (int m = 0; m < WV_M; ++m) p = __builtin_fmaf(vc[64 * m], w[m], p);
-    p = g_sum<64>(p);
-    if (l == 0) lh[c] = p;
+    for (int u = 0; u < 16; ++u) {
+      const float *vc = Vs + (size_t)(c0 + u) * WV_K + l;
+      float p = 0.f;
+      if (c0 + u <= j) {
+#pragma unroll
+        for (int m = 0; m < WV_M; ++m) p = __builtin_fmaf(vc[64 * m], w[m], p);
+      }
+      x[u] = p;
+    }
+    const float r = wave_rs<16, float>(x, l);
+    if ((l & 3) == 0) lh[c0 + (l >> 2)] = r;
   }
   __syncthreads();
   for (int c = 0; c <= j; ++c) {
