@@ -1811,7 +1811,7 @@ hipError_t launch_solve(int group, int pad, int mode, const SolveArgs &a, hipStr
 // resident wavefront (2 x 256 x 256 FP32): column c of G_S (equal to row c)
 // and basis vector c are each a run of 256 floats, read coalesced (lane l
 // takes entries l + 64 m, m < 4) and streamed every step. FP32 matvec and
-// Gram-Schmidt (coefficients by wavefront sums); alpha and beta^2 as FP64
+// Gram-Schmidt (coefficients by wavefront reduce-scatters); alpha and beta^2 as FP64
 // sums of exact products; one classical Gram-Schmidt pass, a second one by
 // the DGKS test; the breakdown rule of the other kernels (restart with the
 // e_i of largest residual, orthogonalised twice). T and the bounds go to
