@@ -342,7 +342,7 @@ pl_status set_hip_error(hipError_t e, const char *what) { return hip_fail(e, wha
 int lds_bytes_of(int form, int k) {
   const Variant v = job_variant(form, k);
   if (v.lanes == 512) return 159 * 1024;  // the row-part kernels' LDS, to the KiB
-  if (v.lanes == 256) return 71 * 1024;
+  if (v.lanes == 256) return 51 * 1024;
   if (v.lanes == 128) return 128 * (128 + v.pad) * 4;
   return v.lanes * (v.lanes + v.pad) * 4;
 }

@@ -982,16 +982,19 @@ __global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
 // row: one workgroup of Q KC lanes per subset. Lane t = (row i = t % KC,
 // part h = t / KC) holds G_S[i][KC/Q h .. KC/Q (h + 1) - 1] in VGPRs. The
 // Lanczos basis V keeps columns 0..L-1 in LDS (row-major, stride S). At
-// KC = 256 an FP32 G_S and an FP32 basis do not both fit one CU's LDS, so
-// columns L..255 live in VGPRs, column L + Q m + h of row i as vr[m] on lane
-// (i, h); at KC = 128 the whole basis is in LDS and two subsets share a CU.
+// KC = 256 an FP32 G_S and an FP32 basis do not both fit one CU's LDS, and at
+// KC = 128 three subsets share a CU, so columns L..KC-1 live in VGPRs, column
+// L + Q m + h of row i as vr[m] on lane (i, h).
 // Gram-Schmidt coefficients of the LDS columns are column dot products (the
 // lanes split into columns x row groups); those of the register columns come
 // from a wavefront reduce-scatter of the lanes' products (wave_rs) and a sum
 // over the wavefronts of each part in LDS. The coefficients are accumulated
 // in T (float, or double: FP64 reorthogonalisation coefficients) and applied
 // in an FP32 update.
-//   KC = 128, Q = 2: 256 lanes, L = 128 (two workgroups per CU)
+//   KC = 128, Q = 2: 256 lanes, L = 92 (three workgroups per CU: a third
+//   wavefront per SIMD hides more LDS latency than the 18 register columns
+//   cost; L = 128 with two workgroups ran k = 128 at 171 us per subset and CU
+//   in FP32, 184 in FP64, against 131 and 169)
 //   KC = 256, Q = 2: 512 lanes, L = 148 (the widest LDS share that fits: each
 //   column moved out of the VGPRs relieves the step loop's register pressure)
 //   Q = 4 (1024 lanes, four wavefronts per SIMD) agreed with Q = 2 at L = 140
@@ -1006,8 +1009,8 @@ struct HR {
   static constexpr int WPH = KC / 64;           // wavefronts per row part
   static constexpr int NW = Q * WPH;            // wavefronts
   static constexpr int LOGK = KC == 256 ? 8 : 7;
-  static constexpr int L = KC == 256 ? 148 : 128;  // basis columns in LDS
-  static constexpr int S = KC == 256 ? 148 : 132;  // LDS row stride: 16-byte rows, stride/4 odd
+  static constexpr int L = KC == 256 ? 148 : 92;  // basis columns in LDS
+  static constexpr int S = KC == 256 ? 148 : 92;  // LDS row stride: 16-byte rows, stride/4 odd
   static constexpr int R = (KC - L) / Q;        // register basis columns per lane
   static constexpr int RA = R > 0 ? R : 1;      // their array extent
   static_assert(S % 4 == 0 && (S / 4) % 2 == 1 && L % 4 == 0 && L <= S, "row layout");
@@ -1067,7 +1070,7 @@ constexpr size_t hr_lds_bytes() {
   return sizeof(HRLds<KC, Q, double>);
 }
 static_assert(hr_lds_bytes<256, 2>() <= 163840, "LDS budget, 512 lanes");
-static_assert(2 * hr_lds_bytes<128, 2>() <= 163840, "LDS budget, 256 lanes: two workgroups per CU");
+static_assert(3 * hr_lds_bytes<128, 2>() <= 163840, "LDS budget, 256 lanes: three workgroups per CU");
 
 template <typename T>
 __device__ __forceinline__ T swap_add(T a, T b, bool wide32);
@@ -1334,11 +1337,11 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   {
     const int g = (int)__umulhi(2u * (unsigned)t, sp.mul), c = t - g * ncl;
     if (g < ng) {
-      // RB rows per iteration (16 FP32; 8 FP64, 4 at KC = 256: register
-      // budget), then 4 at a time: the column entries of a batch are all
-      // loaded before the FMAs consume them, so each batch pays the LDS
-      // latency once
-      constexpr int RB = sizeof(T) == 8 ? (KC == 256 ? 4 : 8) : 16;
+      // RB rows per iteration (register budget: 16 FP32 and 4 FP64 at
+      // KC = 256, 8 at KC = 128 with its 168-VGPR cap), then 4 at a time: the
+      // column entries of a batch are all loaded before the FMAs consume
+      // them, so each batch pays the LDS latency once
+      constexpr int RB = KC == 256 ? (sizeof(T) == 8 ? 4 : 16) : 8;
       const int r0 = g * RR;
       const int rn = (min(RR, max(0, k - r0)) + 3) & ~3;
       // ru = the rows of the last group, the fewest: a wave-uniform bound,
@@ -1699,7 +1702,7 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
 }
 
 template <int KC, int Q, int MODE, typename T>
-__global__ __launch_bounds__(Q * KC) void solve_rowpart_kernel(SolveArgs a) {
+__global__ __launch_bounds__(Q * KC, KC == 128 ? 3 : 1) void solve_rowpart_kernel(SolveArgs a) {
   __shared__ __attribute__((aligned(16))) HRLds<KC, Q, T> lds;
   int64_t cnt = a.nslots;
   const int32_t *ord = a.order;
