@@ -58,6 +58,25 @@ def test_k256_vs_oracle(gpu, k):
         np.testing.assert_allclose(de, np.maximum(hi - 1.0, 1.0 - lo), rtol=1e-6, atol=1e-7)
 
 
+@pytest.mark.parametrize("k", [66, 92, 93, 94, 111, 127, 128])
+def test_256_lane_register_columns_vs_oracle(gpu, k):
+    """64 < k <= 128 on the 256-lane kernel: basis columns 0..91 in LDS, the
+    rest in registers (two per lane and row). k = 93, 94 form the first
+    register column of each row part; every Gram-Schmidt coefficient form
+    (forced FP32, forced FP64, the default) against the oracle."""
+    assert solver.tuning_get(k) == (256, 4)
+    n = 768
+    G = solver.gram(solver.gen_phi("gauss", 300 + k, 4 * k, n))
+    idx = solver.gen_subsets(31 + k, 40, k, n)
+    olo, ohi, _, ost = oracle.solve_batch(G.cpu().numpy(), idx.cpu().numpy(), k=k, nthreads=8)
+    assert np.all(ost == 0)
+    for r in ("f32", "f64", None):
+        lo, hi, de, st = _np(solver.solve(G, idx, k=k, reorth=r))
+        assert np.all(st == 0), r
+        assert rel(lo, olo).max() <= RTOL and rel(hi, ohi).max() <= RTOL, r
+        np.testing.assert_allclose(de, np.maximum(hi - 1.0, 1.0 - lo), rtol=1e-6, atol=1e-7)
+
+
 def test_k256_rank_deficient_alleig(gpu):
     """m = 160 < k: G_S has rank 160 and the Krylov space of the start vector
     is exhausted early. Whether beta then falls under the restart threshold
