@@ -21,6 +21,8 @@ Reported beside it, in the same JSON line:
     plain_multiwave / vs_plain_multiwave: the same against the plain form
     that runs k > 64 on the multi-wavefront kernels (PL_FORM_PLAIN). And
     the tuned batch without the serving engine;
+  * served_f32: the same waves served with FP32 Gram-Schmidt coefficients at
+    every k (the plain forms' arithmetic), against PL_FORM_PLAIN;
   * accuracy: relative error of lambda_min / lambda_max of a 4096-subset
     sample of the served outputs against FP64 LAPACK dsyevd;
   * roofline of the dominant kernel (the k = 256 gather + Lanczos kernel,
@@ -446,6 +448,21 @@ def main():
                         "solves_s": thr, "p50_ms": p50 * 1e3, "p99_ms": p99 * 1e3})
     srv.close()
 
+    # the same waves served with FP32 Gram-Schmidt coefficients at every k: the
+    # arithmetic of the multi-wavefront plain form, to compare the engine and
+    # kernels with it like for like
+    srv = solver.Server(G, max_subsets=16384, max_indices=16384 * 256, max_requests=1024,
+                        max_k=256, depth=3, reorth="f32")
+    serve_wave(srv, reqs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        serve_wave(srv, reqs)
+    torch.cuda.synchronize()
+    f32_wall = time.perf_counter() - t1
+    srv.close()
+    served_f32 = {"value": tot * args.steps / f32_wall, "ms_per_step": f32_wall * 1e3 / args.steps}
+
     # the same subsets as one CSR batch: plain form, and tuned without the engine
     offs = torch.cat([q["off"][1:] - q["off"][:-1] for q in reqs])
     off_all = solver.offsets_from_ks(offs)
@@ -589,6 +606,7 @@ def main():
         "tuned_batch": batch["tuned"],
         "vs_plain": value / batch["plain_wave"]["value"],
         "vs_plain_multiwave": value / batch["plain"]["value"],
+        "served_f32": dict(served_f32, vs_plain_multiwave=served_f32["value"] / batch["plain"]["value"]),
         "accuracy": {"sample": int(len(errs)), "max_rel_err": float(errs.max()),
                      "median_rel_err": float(np.median(errs)), "vs": "FP64 dsyevd",
                      "nonzero_status": int(np.count_nonzero(status))},
