@@ -320,6 +320,9 @@ def main():
                     help="only run the dominant kernel's launches (for rocprof counter runs)")
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the batching and precision configurations")
+    ap.add_argument("--profile-k128", action="store_true",
+                    help="only run stage 1 over the step's k = 128 subsets (the 256-lane kernel, "
+                         "FP64 coefficients) 1 + 3 times, for rocprof")
     ap.add_argument("--profile-gather", action="store_true",
                     help="only run the gather stage over the dominant subsets (rocprof counters)")
     ap.add_argument("--profile-gram", action="store_true",
@@ -358,11 +361,25 @@ def main():
         print(json.dumps({"profile_gram": True, "m": M, "n": N}), flush=True)
         return 0
     G = solver.gram(phi)
-    gram = None if (args.profile_dominant or args.profile_gather) else gram_leg(solver, torch, phi, G)
+    profiling = args.profile_dominant or args.profile_gather or args.profile_k128
+    gram = None if profiling else gram_leg(solver, torch, phi, G)
     torch.cuda.empty_cache()
     reqs = make_requests(solver, torch, args.requests)
     tot = sum(q["nsub"] for q in reqs)
     stream = torch.cuda.current_stream()
+
+    if args.profile_k128:
+        # the second kernel of the mix: stage 1 over as many k = 128 subsets
+        # as one step holds
+        n128 = int(sum(int((q["ks"] == 128).sum().item()) for q in reqs))
+        idx128 = solver.gen_subsets(SEED_SAMPLE + 1, n128, 128, N)
+        ws = torch.empty(solver.workspace_size(n128, 128), dtype=torch.uint8, device=dev)
+        o128 = solver.solve_stage(G, idx128, 1, ws, k=128)
+        for _ in range(3):
+            solver.solve_stage(G, idx128, 1, ws, k=128, out=o128)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_k128": True, "subsets": n128}), flush=True)
+        return 0
 
     # the dominant kernel alone: stage 1 (gather + Lanczos) over as many
     # k = 256 subsets as one step holds, on torch's current stream
