@@ -206,3 +206,36 @@ def test_multisection_agrees_with_bisection(gpu):
         assert not small[3].any() and not big[3][:300].any()
         for a, b in ((small[0], big[0][:300]), (small[1], big[1][:300])):
             assert rel(a, b).max() <= 2.0 ** -22, k
+
+
+@pytest.mark.parametrize("form", ["tuned", "plain_wave"])
+@pytest.mark.parametrize("k", [100, 200])
+def test_edge_cases_large(gpu, k, form):
+    """The edge cases of the small-k kernels at k > 64 (the 256- and 512-lane
+    kernels, and the one-wavefront plain form): identity (a restart at every
+    step), duplicated indices (singular G_S, lambda_min = 0), an out-of-range
+    index and a non-finite entry (flagged, neighbours unaffected)."""
+    n = 512
+    G = torch.eye(n, device="cuda", dtype=torch.float32).contiguous()
+    idx = torch.arange(k, dtype=torch.int32, device="cuda")
+    lo, hi, de, st = _np(solver.solve(G, idx, k=k, form=form))
+    assert st[0] & 1 and (st[0] & ~1) == 0
+    assert abs(lo[0] - 1) < 1e-6 and abs(hi[0] - 1) < 1e-6 and abs(de[0]) < 1e-6
+    G = solver.gram(solver.gen_phi("gauss", 3, 2 * k, n))
+    base = oracle.floyd(5, 0, n, k)
+    dup = base.copy()
+    dup[3] = dup[k - 7]
+    lo, hi, de, st = _np(solver.solve(G, torch.from_numpy(dup).cuda(), k=k, form=form))
+    A, _ = oracle.gather(G.cpu().numpy(), dup)
+    w = np.linalg.eigvalsh(A)
+    assert (st[0] & ~1) == 0
+    assert abs(lo[0] - w[0]) <= 1e-5 * w[-1] and rel(hi[0], w[-1]) <= RTOL
+    two = np.concatenate([base, base]).astype(np.int32)
+    two[k + 2] = n + 5
+    lo, hi, de, st = _np(solver.solve(G, torch.from_numpy(two).cuda(), k=k, form=form))
+    assert st[1] & 8 and np.isnan(lo[1]) and st[0] == 0 and np.isfinite(lo[0])
+    Gn = G.clone()
+    Gn[base[0], base[1]] = float("nan")
+    Gn[base[1], base[0]] = float("nan")
+    lo, hi, de, st = _np(solver.solve(Gn, torch.from_numpy(base).cuda(), k=k, form=form))
+    assert st[0] & 2 and np.isnan(lo[0])
