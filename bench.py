@@ -323,6 +323,9 @@ def main():
     ap.add_argument("--profile-k128", action="store_true",
                     help="only run stage 1 over the step's k = 128 subsets (the 256-lane kernel, "
                          "FP64 coefficients) 1 + 3 times, for rocprof")
+    ap.add_argument("--profile-c2", action="store_true",
+                    help="only run stage 1 and the gather alone of the k = 32 configuration "
+                         "(2^20 subsets of a 4096^2 G), for rocprof")
     ap.add_argument("--profile-gather", action="store_true",
                     help="only run the gather stage over the dominant subsets (rocprof counters)")
     ap.add_argument("--profile-gram", action="store_true",
@@ -349,6 +352,21 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": src.numel() * 4}), flush=True)
+        return 0
+    if args.profile_c2:
+        # the k = 32 configuration: Phi 512 x 4096 (seed 0xC2), 2^20 subsets;
+        # stage 1 (gather + Lanczos, 32-lane groups) and the gather alone,
+        # 1 + 3 launches each
+        G = solver.gram(solver.gen_phi("gauss", 0xC2, 512, 4096))
+        idx = solver.gen_subsets(0xC2 + 1, 1 << 20, 32, 4096)
+        ws = torch.empty(solver.workspace_size(1 << 20, 32), dtype=torch.uint8, device=dev)
+        o = solver.solve_stage(G, idx, 1, ws, k=32)
+        for _ in range(3):
+            solver.solve_stage(G, idx, 1, ws, k=32, out=o)
+        for _ in range(4):
+            solver.gather(G, idx, 32, "checksum", 32, 4)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_c2": True, "subsets": 1 << 20, "k": 32}), flush=True)
         return 0
     t_setup = time.perf_counter()
     phi = solver.gen_phi("dct", SEED_PHI, M, N)
