@@ -8,7 +8,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -255,6 +257,77 @@ hipError_t launch_variant(const Variant &v, const pl::SolveArgs &a, hipStream_t 
 //     k sized to its count, bisection in bin order.
 constexpr int kStageTridiag = 1, kStageBisect = 2;
 
+// Helper stream of a calling stream (one per device and stream, created on
+// first use, default priority, kept until the process ends). A mixed tuned
+// batch runs its widest bin (512 lanes: one workgroup fills a CU) on the
+// caller's stream and the other bins on the helper: their workgroups cannot
+// fit beside the wide kernel's, so they take CUs as its persistent
+// workgroups drain and fill its tail instead of waiting behind it.
+std::mutex g_helper_mu;
+std::map<std::pair<int, hipStream_t>, hipStream_t> g_helper;
+
+hipError_t helper_for(hipStream_t s, hipStream_t *h) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(g_helper_mu);
+  auto it = g_helper.find({dev, s});
+  if (it != g_helper.end()) {
+    *h = it->second;
+    return hipSuccess;
+  }
+  e = hipStreamCreateWithFlags(h, hipStreamNonBlocking);
+  if (e == hipSuccess) g_helper[{dev, s}] = *h;
+  return e;
+}
+
+// the tuned bins of a mixed batch: the widest on s, the rest on the helper
+// stream between a fork after the binning and a join before the bisection
+// (not under graph capture, where everything stays on s)
+hipError_t launch_tuned_bins(const BinPlan &bp, pl::SolveArgs a, hipStream_t s, int64_t nsub,
+                             bool overlap) {
+  int wide = -1;
+  for (int b = 0; b < bp.nbins; ++b)
+    if (bp.var[b].lanes == 512) wide = b;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(s, &cs);
+  if (e != hipSuccess) return e;
+  hipStream_t h = nullptr;
+  const bool fork = overlap && wide >= 0 && bp.nbins > 1 && cs == hipStreamCaptureStatusNone &&
+                    helper_for(s, &h) == hipSuccess;
+  if (!fork) {
+    for (int b = 0; b < bp.nbins; ++b) {
+      a.bin = b;
+      e = pl::launch_solve(bp.var[b].lanes, bp.var[b].pad, pl::MODE_TRIDIAG, a, s, nsub,
+                           bp.var[b].f64);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  hipEvent_t binned = nullptr, rest = nullptr;
+  e = hipEventCreateWithFlags(&binned, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&rest, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(binned, s);
+  if (e == hipSuccess) {
+    a.bin = wide;
+    e = pl::launch_solve(bp.var[wide].lanes, bp.var[wide].pad, pl::MODE_TRIDIAG, a, s, nsub,
+                         bp.var[wide].f64);
+  }
+  if (e == hipSuccess) e = hipStreamWaitEvent(h, binned, 0);
+  for (int b = 0; b < bp.nbins && e == hipSuccess; ++b) {
+    if (b == wide) continue;
+    a.bin = b;
+    e = pl::launch_solve(bp.var[b].lanes, bp.var[b].pad, pl::MODE_TRIDIAG, a, h, nsub,
+                         bp.var[b].f64);
+  }
+  if (e == hipSuccess) e = hipEventRecord(rest, h);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s, rest, 0);
+  // destroying a recorded event releases it once it completes
+  if (binned) (void)hipEventDestroy(binned);
+  if (rest) (void)hipEventDestroy(rest);
+  return e;
+}
+
 pl_status run_job(const SolveJob &j, void *ws, hipStream_t s, int stages = 3) {
   const pl::WorkspaceParts w = pl::split_workspace(ws, j.nsub, j.k);
   pl::SolveArgs a = solve_args(j, w);
@@ -276,12 +349,8 @@ pl_status run_job(const SolveJob &j, void *ws, hipStream_t s, int stages = 3) {
   a.order = w.order;
   if ((j.form & kFormMask) == PL_FORM_TUNED) {
     a.bins = w.table;
-    for (int b = 0; b < bp.nbins; ++b) {
-      a.bin = b;
-      e = pl::launch_solve(bp.var[b].lanes, bp.var[b].pad, pl::MODE_TRIDIAG, a, s, j.nsub,
-                           bp.var[b].f64);
-      if (e != hipSuccess) return hip_fail(e, "persistent solve launch");
-    }
+    e = launch_tuned_bins(bp, a, s, j.nsub, j.overlap_bins);
+    if (e != hipSuccess) return hip_fail(e, "persistent solve launch");
   } else {
     std::vector<int32_t> tab(2 * pl::MAXBIN);
     e = hipMemcpyAsync(tab.data(), w.table, tab.size() * sizeof(int32_t), hipMemcpyDeviceToHost,

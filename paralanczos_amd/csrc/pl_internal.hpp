@@ -149,6 +149,9 @@ struct SolveJob {
   int form;           // PL_FORM_TUNED / PL_FORM_PLAIN (| precision flags), or -1 for an
                       // explicit variant
   Variant explicit_v;  // used when form == -1
+  // tuned mixed batches: the bins other than the widest on a helper stream,
+  // so that they fill the wide kernel's tail
+  bool overlap_bins = true;
 };
 
 // the stream-ordered launches of one solve on a workspace of

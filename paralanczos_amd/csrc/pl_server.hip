@@ -179,6 +179,9 @@ pl_status pl_server::launch(Slot &sl) {
   uint32_t *status = (uint32_t *)(sl.out + 3 * cap);
   pl::SolveJob j{G,   n,     ld,     sl.idx, sl.off, cfg.max_k, sl.tot_sub,
                  lmin, lmax, delta, status, cfg.form, {0, 0, false}};
+  // the steps in flight already overlap on their own streams; helper streams
+  // per step would share the hardware queues with them (measured slower)
+  j.overlap_bins = false;
   pl_status st = pl::solve_job(j, sl.ws, sl.stream);
   if (st != PL_OK) return st;
   const unsigned sblocks = (unsigned)std::min<int64_t>((sl.tot_sub + 255) / 256, 4096);
