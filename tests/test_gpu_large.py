@@ -239,3 +239,31 @@ def test_edge_cases_large(gpu, k, form):
     Gn[base[1], base[0]] = float("nan")
     lo, hi, de, st = _np(solver.solve(Gn, torch.from_numpy(base).cuda(), k=k, form=form))
     assert st[0] & 2 and np.isnan(lo[0])
+
+
+def test_mixed_batch_on_a_side_stream_is_stream_ordered(gpu):
+    """A tuned mixed batch with k > 128 forks its smaller bins to a helper
+    stream and joins them back before the bisection: work queued after the
+    call on the caller's stream sees the complete outputs, repeatedly, on a
+    non-default stream as on the default one."""
+    n = 2048
+    G = solver.gram(solver.gen_phi("gauss", 0xD1, 512, n))
+    ks = solver.gen_ks(0xD2, 900, [16, 64, 128, 256])
+    off = solver.offsets_from_ks(ks)
+    idx = solver.gen_subsets_csr(0xD3, off, n)
+    ref = _np(solver.solve(G, idx, k=256, offsets=off))
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    copies = []
+    with torch.cuda.stream(side):
+        out = solver.solve(G, idx, k=256, offsets=off)
+        for _ in range(3):
+            for t in out:
+                t.fill_(0)
+            solver.solve(G, idx, k=256, offsets=off, out=out)
+            copies.append([t.clone() for t in out])
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    for c in copies:
+        for a, b in zip(_np(c), ref):
+            assert np.array_equal(a, b, equal_nan=True)
