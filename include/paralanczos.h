@@ -60,6 +60,12 @@ typedef enum {
    coefficients at every k > 64 (subsets of k <= 64 always use FP32) */
 #define PL_FORM_REORTH_F32 0x100
 #define PL_FORM_REORTH_F64 0x200
+/* tuned mixed batches: every bin on the caller's stream. Without it, outside
+   graph capture, the bins other than the 512-lane one run on a helper stream
+   (a fixed pool of two per device) between a fork after the binning and a
+   join before the bisection, and fill the wide kernel's tail. Results are
+   the same either way. */
+#define PL_FORM_NO_OVERLAP 0x400
 
 const char *pl_version(void);
 const char *pl_status_string(pl_status st);
@@ -71,6 +77,10 @@ pl_status pl_init(int device);
 
 /* largest subset size the solve kernels accept */
 int pl_max_k(void);
+
+/* diagnostic: mixed tuned batches whose bins were split over the caller's
+   stream and a helper stream since the library was loaded */
+int64_t pl_overlap_forks(void);
 
 /*
  * lambda_min, lambda_max and delta = max(lambda_max - 1, 1 - lambda_min) of
@@ -173,8 +183,10 @@ pl_status pl_tuning_reset(void);
  * requests of nsub * LDS bytes per subset of the kernel for kmax * kmax; 0 =
  * no bound); a request alone in a step is always admitted. Up to `depth`
  * steps run at once, each on its own stream. Results equal those of pl_solve
- * with the same form, whatever the mix. Submission and waiting are
- * thread-safe; pl_server_destroy finishes every queued request first.
+ * with the same form, whatever the mix. A request whose offsets do not run
+ * from 0 to nidx gets status PL_SUB_BADK and NaN outputs for every subset;
+ * its indices never reach another request's subsets. Submission and waiting
+ * are thread-safe; pl_server_destroy finishes every queued request first.
  */
 typedef struct pl_server pl_server;
 typedef struct {

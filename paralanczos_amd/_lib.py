@@ -22,6 +22,7 @@ SIGNATURES = {
     "pl_last_error": (ctypes.c_char_p, []),
     "pl_init": (ST, [INT]),
     "pl_max_k": (INT, []),
+    "pl_overlap_forks": (I64, []),
     "pl_solve": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, P]),
     "pl_solve_workspace_size": (ctypes.c_size_t, [I64, INT]),
     "pl_solve_ws": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, P, ctypes.c_size_t, P]),

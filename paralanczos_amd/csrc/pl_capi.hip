@@ -8,6 +8,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -257,33 +259,50 @@ hipError_t launch_variant(const Variant &v, const pl::SolveArgs &a, hipStream_t 
 //     k sized to its count, bisection in bin order.
 constexpr int kStageTridiag = 1, kStageBisect = 2;
 
-// Helper stream of a calling stream (one per device and stream, created on
-// first use, default priority, kept until the process ends). A mixed tuned
+// Helper streams of mixed tuned batches: a fixed pool of kHelpers streams
+// per device (default priority, created on first use with the calling
+// stream's device current, kept until the process ends); a calling stream
+// takes the pool entry its handle hashes to, so the number of helper streams
+// stays bounded however many calling streams come and go. A mixed tuned
 // batch runs its widest bin (512 lanes: one workgroup fills a CU) on the
 // caller's stream and the other bins on the helper: their workgroups cannot
 // fit beside the wide kernel's, so they take CUs as its persistent
-// workgroups drain and fill its tail instead of waiting behind it.
+// workgroups drain and fill its tail instead of waiting behind it. Batches
+// of two calling streams that share a helper have their small bins ordered
+// after one another on it.
+constexpr int kHelpers = 2;
 std::mutex g_helper_mu;
-std::map<std::pair<int, hipStream_t>, hipStream_t> g_helper;
+std::map<int, std::vector<hipStream_t>> g_helper;
+std::atomic<int64_t> g_forks{0};
 
 hipError_t helper_for(hipStream_t s, hipStream_t *h) {
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
+  hipDevice_t sdev = 0;
+  hipError_t e = hipStreamGetDevice(s, &sdev);
   if (e != hipSuccess) return e;
+  const int dev = (int)sdev;
+  const size_t slot = (std::hash<const void *>{}((const void *)s) >> 4) % kHelpers;
   std::lock_guard<std::mutex> lock(g_helper_mu);
-  auto it = g_helper.find({dev, s});
-  if (it != g_helper.end()) {
-    *h = it->second;
+  std::vector<hipStream_t> &pool = g_helper[dev];
+  if (pool.empty()) pool.assign(kHelpers, nullptr);
+  if (pool[slot]) {
+    *h = pool[slot];
     return hipSuccess;
   }
+  int cur = 0;
+  e = hipGetDevice(&cur);
+  if (e != hipSuccess) return e;
+  if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return e;
   e = hipStreamCreateWithFlags(h, hipStreamNonBlocking);
-  if (e == hipSuccess) g_helper[{dev, s}] = *h;
+  if (cur != dev) (void)hipSetDevice(cur);
+  if (e == hipSuccess) pool[slot] = *h;
   return e;
 }
 
 // the tuned bins of a mixed batch: the widest on s, the rest on the helper
 // stream between a fork after the binning and a join before the bisection
-// (not under graph capture, where everything stays on s)
+// (not under graph capture, where everything stays on s). Once the helper
+// waits on the fork, s always waits on the helper's join, also when a launch
+// failed, so nothing queued on the helper outlives what s orders after it.
 hipError_t launch_tuned_bins(const BinPlan &bp, pl::SolveArgs a, hipStream_t s, int64_t nsub,
                              bool overlap) {
   int wide = -1;
@@ -293,8 +312,20 @@ hipError_t launch_tuned_bins(const BinPlan &bp, pl::SolveArgs a, hipStream_t s, 
   hipError_t e = hipStreamIsCapturing(s, &cs);
   if (e != hipSuccess) return e;
   hipStream_t h = nullptr;
-  const bool fork = overlap && wide >= 0 && bp.nbins > 1 && cs == hipStreamCaptureStatusNone &&
-                    helper_for(s, &h) == hipSuccess;
+  hipEvent_t binned = nullptr, rest = nullptr;
+  bool fork = overlap && wide >= 0 && bp.nbins > 1 && cs == hipStreamCaptureStatusNone &&
+              helper_for(s, &h) == hipSuccess;
+  if (fork) {
+    fork = hipEventCreateWithFlags(&binned, hipEventDisableTiming) == hipSuccess &&
+           hipEventCreateWithFlags(&rest, hipEventDisableTiming) == hipSuccess &&
+           hipEventRecord(binned, s) == hipSuccess && hipStreamWaitEvent(h, binned, 0) == hipSuccess;
+    if (!fork) {
+      // the helper may wait on `binned` already: it orders nothing after
+      // s's binning, and no work goes to the helper
+      if (binned) (void)hipEventDestroy(binned);
+      if (rest) (void)hipEventDestroy(rest);
+    }
+  }
   if (!fork) {
     for (int b = 0; b < bp.nbins; ++b) {
       a.bin = b;
@@ -304,28 +335,24 @@ hipError_t launch_tuned_bins(const BinPlan &bp, pl::SolveArgs a, hipStream_t s, 
     }
     return hipSuccess;
   }
-  hipEvent_t binned = nullptr, rest = nullptr;
-  e = hipEventCreateWithFlags(&binned, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&rest, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventRecord(binned, s);
-  if (e == hipSuccess) {
-    a.bin = wide;
-    e = pl::launch_solve(bp.var[wide].lanes, bp.var[wide].pad, pl::MODE_TRIDIAG, a, s, nsub,
-                         bp.var[wide].f64);
-  }
-  if (e == hipSuccess) e = hipStreamWaitEvent(h, binned, 0);
+  g_forks.fetch_add(1, std::memory_order_relaxed);
+  a.bin = wide;
+  e = pl::launch_solve(bp.var[wide].lanes, bp.var[wide].pad, pl::MODE_TRIDIAG, a, s, nsub,
+                       bp.var[wide].f64);
   for (int b = 0; b < bp.nbins && e == hipSuccess; ++b) {
     if (b == wide) continue;
     a.bin = b;
     e = pl::launch_solve(bp.var[b].lanes, bp.var[b].pad, pl::MODE_TRIDIAG, a, h, nsub,
                          bp.var[b].f64);
   }
-  if (e == hipSuccess) e = hipEventRecord(rest, h);
-  if (e == hipSuccess) e = hipStreamWaitEvent(s, rest, 0);
+  // the join, unconditionally
+  hipError_t f = hipEventRecord(rest, h);
+  if (f == hipSuccess) f = hipStreamWaitEvent(s, rest, 0);
+  if (f != hipSuccess) f = hipStreamSynchronize(h);  // no event: wait for the helper
   // destroying a recorded event releases it once it completes
-  if (binned) (void)hipEventDestroy(binned);
-  if (rest) (void)hipEventDestroy(rest);
-  return e;
+  (void)hipEventDestroy(binned);
+  (void)hipEventDestroy(rest);
+  return e != hipSuccess ? e : f;
 }
 
 pl_status run_job(const SolveJob &j, void *ws, hipStream_t s, int stages = 3) {
@@ -349,7 +376,7 @@ pl_status run_job(const SolveJob &j, void *ws, hipStream_t s, int stages = 3) {
   a.order = w.order;
   if ((j.form & kFormMask) == PL_FORM_TUNED) {
     a.bins = w.table;
-    e = launch_tuned_bins(bp, a, s, j.nsub, j.overlap_bins);
+    e = launch_tuned_bins(bp, a, s, j.nsub, j.overlap_bins && !(j.form & PL_FORM_NO_OVERLAP));
     if (e != hipSuccess) return hip_fail(e, "persistent solve launch");
   } else {
     std::vector<int32_t> tab(2 * pl::MAXBIN);
@@ -382,7 +409,8 @@ pl_status check_job(const SolveJob &j) {
       return fail(PL_ERR_ARG, "no variant with %d lanes and pad %d", j.explicit_v.lanes,
                   j.explicit_v.pad);
   } else if (((j.form & kFormMask) != PL_FORM_TUNED && !is_plain(j.form)) ||
-             (j.form & ~(kFormMask | kPrecMask)) || (j.form & kPrecMask) == kPrecMask) {
+             (j.form & ~(kFormMask | kPrecMask | PL_FORM_NO_OVERLAP)) ||
+             (j.form & kPrecMask) == kPrecMask) {
     return fail(PL_ERR_ARG, "unknown form %d", j.form);
   } else if ((j.form & kFormMask) == PL_FORM_PLAIN_WAVE && (j.form & kPrecMask)) {
     return fail(PL_ERR_ARG, "the one-wavefront form is FP32 only (no precision flags)");
@@ -441,6 +469,8 @@ pl_status pl_init(int device) {
 }
 
 int pl_max_k(void) { return kMaxK; }
+
+int64_t pl_overlap_forks(void) { return g_forks.load(std::memory_order_relaxed); }
 
 size_t pl_solve_workspace_size(int64_t nsub, int k) {
   if (nsub < 0 || k < 1) return 0;

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../../include/paralanczos.h"
+#include "pl_common.hpp"
 #include "pl_internal.hpp"
 
 namespace {
@@ -78,11 +79,20 @@ __global__ void assemble_kernel(const ReqEntry *t, int nreq, int64_t tot_sub, in
     }
     const ReqEntry &r = t[find_entry(t, nreq, s, true)];
     const int64_t l = s - r.sub_base;
-    // clamped into the request's own index range: offsets that disagree with
-    // nidx cannot reach outside it (a decreasing pair gives k < 1: BADK)
-    const int64_t o = r.off ? r.off[l] : l * (int64_t)r.kuni;
+    // clamped into the request's own index range, and the first subset of a
+    // request starts at its own base whatever off[0] holds: a request's
+    // subsets end where the next request's first one starts, so offsets that
+    // disagree with nidx cannot reach into a neighbour's indices (a
+    // decreasing pair gives k < 1: BADK). Malformed requests are flagged as
+    // a whole by the scatter.
+    const int64_t o = (r.off && l > 0) ? r.off[l] : l * (int64_t)r.kuni;
     off[s] = r.idx_base + min(max(o, (int64_t)0), r.nidx);
   }
+}
+
+// request-local offsets that do not run from 0 to nidx
+__device__ __forceinline__ bool malformed(const ReqEntry &r) {
+  return r.off && (r.off[0] != 0 || r.off[r.nsub] != r.nidx);
 }
 
 __global__ void scatter_kernel(const ReqEntry *t, int nreq, int64_t tot_sub, const float *lmin,
@@ -91,6 +101,11 @@ __global__ void scatter_kernel(const ReqEntry *t, int nreq, int64_t tot_sub, con
   for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < tot_sub; s += stride) {
     const ReqEntry &r = t[find_entry(t, nreq, s, true)];
     const int64_t l = s - r.sub_base;
+    if (malformed(r)) {
+      r.lmin[l] = r.lmax[l] = r.delta[l] = NAN;
+      r.status[l] = pl::ST_BADK;
+      continue;
+    }
     r.lmin[l] = lmin[s];
     r.lmax[l] = lmax[s];
     r.delta[l] = delta[s];

@@ -15,6 +15,7 @@ from ._lib import ParaLanczosError, check, load
 
 FORMS = {"tuned": 0, "plain": 1, "plain_wave": 2}
 REORTH = {None: 0, "auto": 0, "f32": 0x100, "f64": 0x200}
+NO_OVERLAP = 0x400
 PHI_KINDS = {"gauss": 0, "bernoulli": 1, "dct": 2}
 
 SUB_BREAKDOWN = 1
@@ -130,11 +131,12 @@ def subgram(phi: torch.Tensor, idx: torch.Tensor, k: int, layout: str = "col"):
 
 
 def solve(G, idx, k=None, offsets=None, form="tuned", lanes=None, pad=None, out=None,
-          reorth=None):
+          reorth=None, overlap=True):
     """lambda_min, lambda_max, delta (FP32) and status (int32 view of the u32
     status word) per subset. Uniform size k, or CSR offsets (k = largest size).
     reorth: None/'auto' (FP64 Gram-Schmidt coefficients from k = 128 on in the
-    tuned form), 'f32' or 'f64' (forced for k > 64)."""
+    tuned form), 'f32' or 'f64' (forced for k > 64). overlap=False keeps every
+    bin of a tuned mixed batch on the current stream (no helper stream)."""
     L = load()
     _dev(G, torch.float32, "G")
     _dev(idx, torch.int32, "idx")
@@ -162,8 +164,15 @@ def solve(G, idx, k=None, offsets=None, form="tuned", lanes=None, pad=None, out=
         pad = pad if pad is not None else 4
         check(L.pl_solve_variant(*args, int(lanes), int(pad), _stream()), "pl_solve_variant")
     else:
-        check(L.pl_solve(*args, FORMS[form] | REORTH[reorth], _stream()), "pl_solve")
+        flags = FORMS[form] | REORTH[reorth] | (0 if overlap else NO_OVERLAP)
+        check(L.pl_solve(*args, flags, _stream()), "pl_solve")
     return lmin, lmax, delta, status
+
+
+def overlap_forks() -> int:
+    """Mixed tuned batches split over the caller's stream and a helper stream
+    since the library was loaded."""
+    return int(load().pl_overlap_forks())
 
 
 def workspace_size(nsub, k):

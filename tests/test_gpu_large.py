@@ -245,13 +245,21 @@ def test_mixed_batch_on_a_side_stream_is_stream_ordered(gpu):
     """A tuned mixed batch with k > 128 forks its smaller bins to a helper
     stream and joins them back before the bisection: work queued after the
     call on the caller's stream sees the complete outputs, repeatedly, on a
-    non-default stream as on the default one."""
+    non-default stream as on the default one. The fork is checked to have
+    happened (the library's fork counter), and the outputs equal those of the
+    same batch with every bin on the caller's stream."""
     n = 2048
     G = solver.gram(solver.gen_phi("gauss", 0xD1, 512, n))
     ks = solver.gen_ks(0xD2, 900, [16, 64, 128, 256])
     off = solver.offsets_from_ks(ks)
     idx = solver.gen_subsets_csr(0xD3, off, n)
+    f0 = solver.overlap_forks()
+    seq = _np(solver.solve(G, idx, k=256, offsets=off, overlap=False))
+    assert solver.overlap_forks() == f0
     ref = _np(solver.solve(G, idx, k=256, offsets=off))
+    assert solver.overlap_forks() == f0 + 1
+    for a, b in zip(ref, seq):
+        assert np.array_equal(a, b, equal_nan=True)
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     copies = []
@@ -264,6 +272,7 @@ def test_mixed_batch_on_a_side_stream_is_stream_ordered(gpu):
             copies.append([t.clone() for t in out])
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
+    assert solver.overlap_forks() == f0 + 5
     for c in copies:
-        for a, b in zip(_np(c), ref):
+        for a, b in zip(_np(c), seq):
             assert np.array_equal(a, b, equal_nan=True)

@@ -120,3 +120,33 @@ def test_uniform_requests_and_precision_form(env):
                 assert np.array_equal(a, b), k
     finally:
         srv.close()
+
+
+def test_malformed_request_stays_in_its_own_results(env):
+    """Requests whose offsets do not run from 0 to nidx (a slice of a larger
+    CSR batch: off[0] > 0; a truncated one: off[nsub] < nidx) are flagged
+    BADK with NaN outputs as a whole, and the requests packed beside them in
+    the same step keep their results bit for bit."""
+    G, reqs = env
+    good = [reqs[r] for r in (2, 5, 9)]
+    direct = [_np(solver.solve(G, idx, k=256, offsets=off)) for idx, off, _, _ in good]
+    shifted = reqs[3][1] + 2                      # off[0] = 2, off[nsub] = nidx + 2
+    trunc = reqs[4][1].clone()
+    trunc[-1] -= 1                                # off[nsub] = nidx - 1
+    bad = [(reqs[3][0], shifted, reqs[3][2]), (reqs[4][0], trunc, reqs[4][2])]
+    srv = solver.Server(G, max_subsets=4096, max_indices=4096 * 256, depth=1)
+    try:
+        tickets = [srv.submit(good[0][0], offsets=good[0][1], kmax=good[0][2]),
+                   srv.submit(bad[0][0], offsets=bad[0][1], kmax=bad[0][2]),
+                   srv.submit(good[1][0], offsets=good[1][1], kmax=good[1][2]),
+                   srv.submit(bad[1][0], offsets=bad[1][1], kmax=bad[1][2]),
+                   srv.submit(good[2][0], offsets=good[2][1], kmax=good[2][2])]
+        outs = [_np(srv.wait(t)[0]) for t in tickets]
+    finally:
+        srv.close()
+    for got, ref in zip((outs[0], outs[2], outs[4]), direct):
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+    for got in (outs[1], outs[3]):
+        assert np.all(got[3] == solver.SUB_BADK)
+        assert np.all(np.isnan(got[0])) and np.all(np.isnan(got[1])) and np.all(np.isnan(got[2]))
