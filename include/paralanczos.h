@@ -114,6 +114,23 @@ pl_status pl_solve_ws(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                       float *delta, uint32_t *status, int form, void *workspace,
                       size_t workspace_bytes, void *stream);
 
+/* pl_solve without any device allocation. No counterpart in the reference:
+   it is for callers that cannot allocate (code under HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc). The workspace is a
+   library-owned device array of PL_NOALLOC_SCRATCH_BYTES per GPU, held from
+   library load until the process ends; calls with
+   pl_solve_workspace_size(nsub, k) above that are rejected (PL_ERR_ARG).
+   Every call on a GPU uses that GPU's one array, so calls that overlap on one
+   GPU (other streams, other threads) race: order them. Tuned mixed batches
+   run every bin on `stream` (as with PL_FORM_NO_OVERLAP); PL_FORM_PLAIN_WAVE
+   is not accepted (PL_ERR_UNSUPPORTED); PL_FORM_PLAIN with offsets waits on
+   the host for its bin counts. Results equal pl_solve's. */
+#define PL_NOALLOC_SCRATCH_BYTES (32u << 20)
+pl_status pl_solve_noalloc(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                           const int64_t *offsets, int k, int64_t nsub, float *lmin,
+                           float *lmax, float *delta, uint32_t *status, int form,
+                           void *stream);
+
 /* One stage of pl_solve_ws on a caller workspace: stage 1 gathers each G_S
    and runs the Lanczos recurrence (the tridiagonal T of every subset and, for
    a mixed batch, its bins stay in the workspace); stage 2 bisects T for

@@ -25,6 +25,7 @@ SIGNATURES = {
     "pl_overlap_forks": (I64, []),
     "pl_solve": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, P]),
     "pl_solve_workspace_size": (ctypes.c_size_t, [I64, INT]),
+    "pl_solve_noalloc": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, P]),
     "pl_solve_ws": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, P, ctypes.c_size_t, P]),
     "pl_solve_stage": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, INT, P,
                             ctypes.c_size_t, P]),

@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <functional>
 #include <map>
@@ -379,7 +380,7 @@ pl_status run_job(const SolveJob &j, void *ws, hipStream_t s, int stages = 3) {
     e = launch_tuned_bins(bp, a, s, j.nsub, j.overlap_bins && !(j.form & PL_FORM_NO_OVERLAP));
     if (e != hipSuccess) return hip_fail(e, "persistent solve launch");
   } else {
-    std::vector<int32_t> tab(2 * pl::MAXBIN);
+    std::array<int32_t, 2 * pl::MAXBIN> tab;
     e = hipMemcpyAsync(tab.data(), w.table, tab.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
                        s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -429,6 +430,40 @@ pl_status solve_once(const SolveJob &j, void *ws, size_t ws_bytes, void *stream)
   return run_job(j, w.p, s);
 }
 
+// ---------------------------------------------------------------------------
+// The workspace of pl_solve_noalloc: a file-scope device array (each device
+// holds its own copy from module load until the process ends). Host code
+// never uses the array's name as an address: the address of each device's
+// copy comes from hipGetSymbolAddress, resolved once per device with that
+// device current (by pl_init, or by the first pl_solve_noalloc on it).
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = (size_t)32 << 20;  // PL_NOALLOC_SCRATCH_BYTES
+static_assert(kNoAllocBytes == PL_NOALLOC_SCRATCH_BYTES, "header and library agree");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// this device's copy of `scratch` (the current device)
+pl_status noalloc_scratch(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(PL_ERR_UNSUPPORTED, "device %d >= %d devices", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return fail(PL_ERR_HIP, "hipGetSymbolAddress(scratch) gave a null address");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return PL_OK;
+}
+
 }  // namespace
 
 namespace pl {
@@ -465,7 +500,8 @@ const char *pl_last_error(void) { return g_err.c_str(); }
 pl_status pl_init(int device) {
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-  return PL_OK;
+  void *p = nullptr;
+  return noalloc_scratch(&p);
 }
 
 int pl_max_k(void) { return kMaxK; }
@@ -490,6 +526,30 @@ pl_status pl_solve_ws(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                       size_t workspace_bytes, void *stream) {
   SolveJob j{G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, form, {0, 0, false}};
   return solve_once(j, workspace, workspace_bytes, stream);
+}
+
+pl_status pl_solve_noalloc(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                           const int64_t *offsets, int k, int64_t nsub, float *lmin,
+                           float *lmax, float *delta, uint32_t *status, int form,
+                           void *stream) {
+  SolveJob j{G, n, ld, idx, offsets, k, nsub, lmin, lmax, delta, status, form, {0, 0, false}};
+  j.overlap_bins = false;  // no helper stream or events to create
+  pl_status st = check_job(j);
+  if (st != PL_OK) return st;
+  if ((form & kFormMask) == PL_FORM_PLAIN_WAVE)
+    return fail(PL_ERR_UNSUPPORTED,
+                "pl_solve_noalloc: the one-wavefront plain form allocates its own scratch");
+  const size_t need = pl::workspace_bytes(nsub, k);
+  if (need > kNoAllocBytes)
+    return fail(PL_ERR_ARG,
+                "pl_solve_noalloc: %lld subsets of k <= %d need %zu bytes of scratch > %zu "
+                "(PL_NOALLOC_SCRATCH_BYTES)",
+                (long long)nsub, k, need, kNoAllocBytes);
+  void *ws = nullptr;
+  st = noalloc_scratch(&ws);
+  if (st != PL_OK) return st;
+  if (nsub == 0) return PL_OK;
+  return run_job(j, ws, (hipStream_t)stream);
 }
 
 pl_status pl_solve_stage(const float *G, int64_t n, int64_t ld, const int32_t *idx,
