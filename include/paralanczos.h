@@ -15,6 +15,8 @@
  *   pl_solve_alleig   - the same kernels' full outputs Eigenvalues[s*k + j]
  *                       ascending, DetMat[s], Eigerror[s]
  *                       (ParaLanczos.cu:1465-1478).
+ *   pl_upper          - UPPER_KER (ParaLanczos.h:36, ParaLanczos.cu:1484-1626),
+ *                       per subset instead of one full matrix.
  *   pl_gen_*          - no reference counterpart: seeded index sets and
  *                       synthetic sensing matrices (the reference takes a
  *                       caller-supplied subIndex, ParaLanczos.cu:1811).
@@ -234,6 +236,18 @@ pl_status pl_server_wait(pl_server *server, int64_t ticket, double *latency_s, d
 pl_status pl_server_poll(pl_server *server, int64_t ticket, int *done);
 pl_status pl_server_query(pl_server *server, pl_server_stats *stats, int reset);
 pl_status pl_server_destroy(pl_server *server);
+
+/* Power-iteration estimate of lambda_max per subset (the reference's
+   UPPER_KER, ParaLanczos.h:36, ParaLanczos.cu:1484-1626, which runs 10 steps
+   on one full FP64 matrix of n <= 128): from v = ones/sqrt(k), `iters` times
+   w = G_S v, b = ||w||, v = w / b; upper[s] = the last b (FP32; FP32 products,
+   ||w||^2 summed in FP64), a lower bound of lambda_max that approaches it.
+   Subsets as in pl_solve (uniform k, or CSR offsets with k the largest size);
+   status gets PL_SUB_BADINDEX / BADK / NONFINITE with NaN in upper. A full
+   matrix is the identity index set 0..n-1. */
+pl_status pl_upper(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                   const int64_t *offsets, int k, int64_t nsub, int iters, float *upper,
+                   uint32_t *status, void *stream);
 
 /* all k eigenvalues ascending eig[s*k + j] (FP64), det[s] = prod_j eig, and
    err[s] = per-subset status, for nsub subsets of size k (uniform) */

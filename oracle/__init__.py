@@ -74,6 +74,8 @@ def _declare(L):
         "or_solve_batch": (None, [P, I64, I64, P, P, I32, I64, P, P, P, P, I32]),
         "or_solve_batch_alleig": (None, [P, I64, I64, P, I32, I64, P, P, P, I32]),
         "or_solve_batch_faithful": (None, [P, I64, P, I32, I64, P, P, P, I32]),
+        "or_upper": (DBL, [P, I32, I32]),
+        "or_upper_batch": (None, [P, I64, I64, P, P, I32, I64, I32, P, P, I32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -240,3 +242,29 @@ def solve_batch_faithful(G, idx, k, nthreads=1):
     lib().or_solve_batch_faithful(_p(G), n, _p(ix), k, nsub, _p(eig), _p(det), _p(ierr),
                                   nthreads)
     return eig, det, ierr
+
+
+def upper(A, iters=10):
+    """Power-iteration estimate of lambda_max of a dense matrix (FP64)."""
+    A = _c(A, np.float64)
+    return float(lib().or_upper(_p(A), A.shape[0], int(iters)))
+
+
+def upper_batch(G, idx, k=None, offsets=None, iters=10, nthreads=1, ld=None):
+    """Per-subset power-iteration estimates (FP64) and status."""
+    G = _c(G, np.float32)
+    n = G.shape[0]
+    ld = G.shape[1] if ld is None else ld
+    ix = _c(idx, np.int32)
+    if offsets is not None:
+        off = _c(offsets, np.int64)
+        nsub = len(off) - 1
+        offp, kk = _p(off), 0
+    else:
+        nsub = len(ix) // k
+        offp, kk = None, k
+    up = np.zeros(nsub)
+    st = np.zeros(nsub, np.uint32)
+    lib().or_upper_batch(_p(G), n, ld, _p(ix), offp, kk, nsub, int(iters), _p(up), _p(st),
+                         nthreads)
+    return up, st

@@ -19,6 +19,10 @@
  *                       a + 2b cos(j pi/(k+1)), scipy eigvalsh_tridiagonal
  *                       fixtures, brute-force counts on tiny T.
  *   or_tqlrat         - the same closed form and fixtures.
+ *   or_upper          - closed forms: a diagonal matrix (||D^L 1|| /
+ *                       ||D^(L-1) 1||), a rank-one matrix (exact after one
+ *                       step), a multiple of the identity; bounded above by
+ *                       numpy's largest eigenvalue and converging to it.
  */
 #include "oracle.h"
 
@@ -549,5 +553,50 @@ void or_solve_batch_faithful(const double *G, int64_t n, const int32_t *idx, int
       det[s] = prod;
     }
     free(A); free(al); free(be);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Power-iteration estimate of lambda_max, UPPER_KER (ParaLanczos.cu:       */
+/* 1484-1626): v normalised, then `iters` times w = A v, b = ||w||,         */
+/* v = w / b; the estimate is the last b. Substitutions, as in the Lanczos  */
+/* path: start vector ones/sqrt(k) instead of clock()-seeded curand normals */
+/* (:1486-1512), any k <= 256 and principal submatrices instead of one full */
+/* n <= 128 matrix. Not reproduced: the reference's norm sums every lane's  */
+/* copy of w[0..3] (:1591-1599) and its next vector repeats w[0..3] in      */
+/* every lane's chunk (:1605-1616), so it does not compute ||A v||.        */
+/* ------------------------------------------------------------------------ */
+double or_upper(const double *A, int k, int iters) {
+  double *v = (double *)malloc(sizeof(double) * (size_t)k);
+  double *w = (double *)malloc(sizeof(double) * (size_t)k);
+  const double v0 = 1.0 / sqrt((double)k);
+  for (int i = 0; i < k; ++i) v[i] = v0;
+  double b = 0.0;
+  for (int l = 0; l < iters; ++l) {
+    for (int i = 0; i < k; ++i) w[i] = dotd(A + (int64_t)i * k, v, k);
+    b = sqrt(dotd(w, w, k));
+    for (int i = 0; i < k; ++i) v[i] = b > 0.0 ? w[i] / b : 0.0;
+  }
+  free(v); free(w);
+  return b;
+}
+
+void or_upper_batch(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                    const int64_t *offsets, int kuni, int64_t nsub, int iters, double *upper,
+                    uint32_t *status, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+  {
+    double *A = (double *)malloc(sizeof(double) * 256 * 256);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t s = 0; s < nsub; ++s) {
+      int64_t o = offsets ? offsets[s] : s * (int64_t)kuni;
+      int k = offsets ? (int)(offsets[s + 1] - offsets[s]) : kuni;
+      uint32_t st = (k < 1 || k > 256) ? OR_ST_BADK : 0u;
+      if (!st) st |= or_gather(G, n, ld, idx + o, k, A);
+      upper[s] = st ? NAN : or_upper(A, k, iters);
+      status[s] = st;
+    }
+    free(A);
   }
 }

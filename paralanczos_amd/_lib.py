@@ -31,6 +31,7 @@ SIGNATURES = {
                             ctypes.c_size_t, P]),
     "pl_solve_variant": (ST, [P, I64, I64, P, P, INT, I64, P, P, P, P, INT, INT, P]),
     "pl_solve_alleig": (ST, [P, I64, I64, P, INT, I64, P, P, P, P]),
+    "pl_upper": (ST, [P, I64, I64, P, P, INT, I64, INT, P, P, P]),
     "pl_plan_create": (ST, [P, P, I64, I64, P, P, INT, I64, P, P, P, P, INT]),
     "pl_plan_execute": (ST, [P, P]),
     "pl_plan_is_graph": (INT, [P]),

@@ -552,6 +552,30 @@ pl_status pl_solve_noalloc(const float *G, int64_t n, int64_t ld, const int32_t 
   return run_job(j, ws, (hipStream_t)stream);
 }
 
+pl_status pl_upper(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                   const int64_t *offsets, int k, int64_t nsub, int iters, float *upper,
+                   uint32_t *status, void *stream) {
+  pl_status st = check_solve_args(G, n, ld, idx, k, nsub);
+  if (st != PL_OK) return st;
+  if (!upper || !status) return fail(PL_ERR_ARG, "output pointers must be non-null");
+  if (iters < 1 || iters > 1000000)
+    return fail(PL_ERR_ARG, "iters = %d outside [1, 1000000]", iters);
+  if (nsub == 0) return PL_OK;
+  pl::SolveArgs a{};
+  a.G = G;
+  a.n = n;
+  a.ld = ld;
+  a.idx = idx;
+  a.offsets = offsets;
+  a.kuni = offsets ? 0 : k;
+  a.kcap = k;
+  a.nslots = nsub;
+  a.status = status;
+  hipError_t e = pl::launch_upper(a, iters, upper, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "power-iteration launch");
+  return PL_OK;
+}
+
 pl_status pl_solve_stage(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                          const int64_t *offsets, int k, int64_t nsub, float *lmin, float *lmax,
                          float *delta, uint32_t *status, int form, int stage, void *workspace,

@@ -105,6 +105,9 @@ inline size_t wave_scratch_bytes(int64_t slots) {
   return (size_t)slots * 2 * 256 * 256 * sizeof(float);
 }
 hipError_t launch_wave(const SolveArgs &a, float *scratch, int64_t slots, hipStream_t s);
+// power-iteration estimate of lambda_max per subset (iters steps from
+// ones/sqrt(k)) into upper[s], status into a.status; kernel capacity by a.kcap
+hipError_t launch_upper(const SolveArgs &a, int iters, float *upper, hipStream_t s);
 // largest k a variant with this many lanes per subset takes
 int variant_capacity(int group);
 

@@ -2044,6 +2044,129 @@ hipError_t launch_wave(const SolveArgs &a, float *scratch, int64_t slots, hipStr
 }
 
 // ---------------------------------------------------------------------------
+// Power-iteration estimate of lambda_max per subset: the counterpart of
+// UPPER_KER (ParaLanczos.cu:1484-1626), which runs 10 steps of w = A v,
+// b = ||w||, v = w / b on one full n <= 128 matrix held in per-lane 4 x 4
+// register tiles and returns the last b. Here: any principal submatrix with
+// k <= KC (64, 128 or 256), ones/sqrt(k) as the start vector, `iters` steps.
+// One workgroup per subset of Q = KC / 64 row parts (KC Q lanes): lane (i, h)
+// holds G_S[i][64 h + c], c < 64, in VGPRs; v is broadcast from LDS; the row
+// parts' partial dots meet in LDS; ||w||^2 sums the exact FP32 squares in
+// FP64 over the wavefronts of part 0. Three barriers per step.
+// ---------------------------------------------------------------------------
+template <int KC>
+__global__ __launch_bounds__(KC * (KC / 64)) void upper_kernel(SolveArgs a, int iters,
+                                                               float *upper) {
+  constexpr int Q = KC / 64, NT = KC * Q, WP = KC / 64;  // parts, lanes, waves of a part
+  __shared__ __attribute__((aligned(16))) float lv[KC];
+  __shared__ float part[Q][KC];
+  __shared__ double rn[WP];
+  __shared__ int lidx[KC];
+  __shared__ int flag[NT / 64];
+  const int t = threadIdx.x, i = t % KC, h = t / KC, q = t & 63;
+  for (int64_t s = blockIdx.x; s < a.nslots; s += gridDim.x) {
+    int k;
+    int64_t off;
+    if (a.offsets) {
+      off = a.offsets[s];
+      k = (int)(a.offsets[s + 1] - off);
+    } else {
+      k = a.kuni;
+      off = s * (int64_t)k;
+    }
+    const bool badk = k < 1 || k > KC || k > a.kcap;
+    uint32_t st = badk ? ST_BADK : 0u;
+    int my = 0;
+    if (!st && i < k) {
+      my = a.idx[off + i];
+      if (my < 0 || (int64_t)my >= a.n) st = ST_BADINDEX;
+    }
+    if (h == 0) lidx[i] = my;
+    {
+      const bool any = __ballot(st != 0) != 0;
+      if (q == 0) flag[t >> 6] = any ? 1 : 0;
+    }
+    __syncthreads();
+    int anyb = 0;
+#pragma unroll
+    for (int u = 0; u < NT / 64; ++u) anyb |= flag[u];
+    if (anyb) st = badk ? ST_BADK : ST_BADINDEX;
+    float g[64];
+    bool fin = true;
+#pragma unroll
+    for (int c = 0; c < 64; ++c) {
+      const int col = 64 * h + c;
+      float x = 0.f;
+      if (st == 0 && i < k && col < k) {
+        const int ic = lidx[col];
+        const int64_t lo = min(ic, my), hi = max(ic, my);
+        x = a.G[lo * a.ld + hi];
+      }
+      g[c] = x;
+      fin = fin && isfinite(x);
+    }
+    __syncthreads();  // lidx is reused by the next subset
+    {
+      const bool bad = __ballot(!fin) != 0;
+      if (q == 0) flag[t >> 6] = bad ? 1 : 0;
+    }
+    if (h == 0) lv[i] = (i < k) ? 1.0f / sqrtf((float)k) : 0.f;
+    __syncthreads();
+    anyb = 0;
+#pragma unroll
+    for (int u = 0; u < NT / 64; ++u) anyb |= flag[u];
+    if (anyb && st == 0) st = ST_NONFINITE;
+    float b = 0.f;
+    if (st == 0) {
+      for (int l = 0; l < iters; ++l) {
+        float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f;
+        const float4 *v4 = reinterpret_cast<const float4 *>(lv + 64 * h);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+          const float4 x = v4[c];
+          d0 = __builtin_fmaf(g[4 * c + 0], x.x, d0);
+          d1 = __builtin_fmaf(g[4 * c + 1], x.y, d1);
+          d2 = __builtin_fmaf(g[4 * c + 2], x.z, d2);
+          d3 = __builtin_fmaf(g[4 * c + 3], x.w, d3);
+        }
+        part[h][i] = (d0 + d1) + (d2 + d3);
+        __syncthreads();
+        float w = part[0][i];
+#pragma unroll
+        for (int u = 1; u < Q; ++u) w += part[u][i];
+        const double sq = wave_sum_f64((double)w * (double)w);
+        if (h == 0 && q == 0) rn[t >> 6] = sq;
+        __syncthreads();
+        double tot = rn[0];
+#pragma unroll
+        for (int u = 1; u < WP; ++u) tot += rn[u];
+        b = (float)sqrt(tot);
+        if (h == 0) lv[i] = b > 0.f ? w / b : 0.f;
+        __syncthreads();
+      }
+      if (!isfinite(b)) st = ST_NONFINITE;
+    }
+    if (t == 0) {
+      upper[s] = (st & (ST_BADK | ST_BADINDEX | ST_NONFINITE)) ? NAN : b;
+      a.status[s] = st;
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_upper(const SolveArgs &a, int iters, float *upper, hipStream_t s) {
+  if (a.nslots <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<int64_t>(a.nslots, 1 << 16);
+  if (a.kcap <= 64)
+    hipLaunchKernelGGL(upper_kernel<64>, dim3(blocks), dim3(64), 0, s, a, iters, upper);
+  else if (a.kcap <= 128)
+    hipLaunchKernelGGL(upper_kernel<128>, dim3(blocks), dim3(256), 0, s, a, iters, upper);
+  else
+    hipLaunchKernelGGL(upper_kernel<256>, dim3(blocks), dim3(1024), 0, s, a, iters, upper);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Binning of subsets by size: count per bin (LDS histogram per block, one
 // global atomic per bin and block), exclusive scan, scatter of subset ids into
 // the order array (LDS atomics give each subset its place within the block's

@@ -175,6 +175,28 @@ def overlap_forks() -> int:
     return int(load().pl_overlap_forks())
 
 
+def upper(G, idx, k=None, offsets=None, iters=10):
+    """Power-iteration estimate of lambda_max per subset (FP32; the
+    reference's UPPER_KER runs iters = 10) and the status (int32)."""
+    L = load()
+    _dev(G, torch.float32, "G")
+    _dev(idx, torch.int32, "idx")
+    if offsets is not None:
+        _dev(offsets, torch.int64, "offsets")
+        nsub = offsets.numel() - 1
+        if k is None:
+            k = int((offsets[1:] - offsets[:-1]).max().item()) if nsub > 0 else 1
+    else:
+        if k is None:
+            raise ParaLanczosError("k is required without offsets")
+        nsub = idx.numel() // k
+    up = torch.empty(nsub, dtype=torch.float32, device=G.device)
+    status = torch.empty(nsub, dtype=torch.int32, device=G.device)
+    check(L.pl_upper(_p(G), G.shape[0], G.stride(0), _p(idx), _p(offsets), int(k), nsub,
+                     int(iters), _p(up), _p(status), _stream()), "pl_upper")
+    return up, status
+
+
 def workspace_size(nsub, k):
     return int(load().pl_solve_workspace_size(int(nsub), int(k)))
 

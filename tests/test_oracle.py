@@ -269,3 +269,75 @@ def test_planted_mistakes_are_caught():
     Gn = rng.normal(size=(20, 20)).astype(np.float32)
     A1, _ = oracle.gather(Gn, np.array([2, 9, 4], np.int32))
     assert not np.array_equal(A1, Gn[np.ix_([2, 9, 4], [2, 9, 4])].T) or np.allclose(Gn, Gn.T)
+
+
+# ---- power-iteration estimate (UPPER_KER, ParaLanczos.cu:1484-1626) ----
+@pytest.mark.parametrize("iters", [1, 2, 3, 10, 12])
+def test_upper_diagonal_closed_form(iters):
+    """v_0 = 1/sqrt(k): after L steps v is D^L 1 normalised, so the last
+    ||D v|| is ||D^L 1|| / ||D^(L-1) 1||."""
+    d = np.array([3.0, 1.0, 0.5, 2.0, 2.9, 0.1])
+    expect = math.sqrt(np.sum(d ** (2 * iters)) / np.sum(d ** (2 * (iters - 1))))
+    assert abs(oracle.upper(np.diag(d), iters) - expect) <= 1e-13 * expect
+
+
+def test_upper_rank_one_and_identity():
+    """A = u u^T: the first step gives ||u|| |sum u| / sqrt(k), every later
+    one ||u||^2 exactly; c I gives c."""
+    u = np.array([0.5, -1.0, 2.0, 0.25, 1.5])
+    A = np.outer(u, u)
+    k = len(u)
+    assert abs(oracle.upper(A, 1) - np.linalg.norm(u) * abs(u.sum()) / math.sqrt(k)) < 1e-13
+    for L in (2, 5, 10):
+        assert abs(oracle.upper(A, L) - u @ u) < 1e-12
+    assert abs(oracle.upper(2.5 * np.eye(7), 10) - 2.5) < 1e-14
+
+
+def test_upper_bounded_by_and_converges_to_lambda_max():
+    rng = np.random.default_rng(5)
+    for k in (8, 40, 128):
+        X = rng.normal(size=(2 * k, k))
+        A = X.T @ X / (2 * k)
+        w = np.linalg.eigvalsh(A)
+        assert oracle.upper(A, 10) <= w[-1] * (1 + 1e-14)
+        assert abs(oracle.upper(A, 20000) - w[-1]) <= 1e-9 * w[-1]
+
+
+def test_upper_batch_matches_gathered_subsets():
+    rng = np.random.default_rng(8)
+    X = rng.normal(size=(48, 120)).astype(np.float32)
+    G = (X.T @ X).astype(np.float32)
+    G = ((G + G.T) * np.float32(0.5)).astype(np.float32)
+    ks = np.array([5, 17, 1, 33], np.int64)
+    off = np.concatenate([[0], np.cumsum(ks)])
+    idx = oracle.gen_subsets_csr(3, off, 120)
+    up, st = oracle.upper_batch(G, idx, offsets=off, iters=10, nthreads=2)
+    assert np.all(st == 0)
+    for s in range(4):
+        A, _ = oracle.gather(G, idx[off[s]:off[s + 1]])
+        assert up[s] == oracle.upper(A, 10)
+    bad = idx.copy()
+    bad[0] = 500
+    up2, st2 = oracle.upper_batch(G, bad, offsets=off)
+    assert st2[0] == oracle.ST_BADINDEX and np.isnan(up2[0]) and st2[1] == 0
+
+
+def test_upper_planted_mistakes_are_caught():
+    """The closed forms reject an iteration count off by one and a returned
+    squared norm (planted here in a numpy restatement, not in the oracle)."""
+    d = np.array([3.0, 1.0, 0.5, 2.0])
+
+    def planted(A, iters, extra=0, square=False):
+        v = np.ones(len(A)) / math.sqrt(len(A))
+        b = 0.0
+        for _ in range(iters + extra):
+            w = A @ v
+            b = w @ w if square else math.sqrt(w @ w)
+            v = w / math.sqrt(w @ w)
+        return b
+
+    expect = math.sqrt(np.sum(d ** 20) / np.sum(d ** 18))
+    assert abs(planted(np.diag(d), 10) - expect) < 1e-12
+    assert abs(planted(np.diag(d), 10, extra=1) - expect) > 1e-6
+    assert abs(planted(np.diag(d), 10, extra=-1) - expect) > 1e-6
+    assert abs(planted(np.diag(d), 10, square=True) - expect) > 1e-3
