@@ -96,16 +96,18 @@ def serve_wave(srv, reqs):
     return [srv.wait(t) for t in tickets]
 
 
-def poisson_run(srv, reqs, rate, seed):
-    """Submit the requests at Poisson arrival times of the given rate
-    (requests/s); returns (subsets/s over the run, p50, p99 latency in s)."""
+def poisson_run(srv, reqs, rate, seed, passes=5):
+    """Submit `passes` rounds of the requests at Poisson arrival times of the
+    given rate (requests/s); returns (subsets/s over the run, p50, p99 latency
+    in s)."""
     rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / rate, size=len(reqs))
+    arrivals = reqs * passes
+    gaps = rng.exponential(1.0 / rate, size=len(arrivals))
     srv.stats(reset=True)
     t0 = time.perf_counter()
     due = t0
     tickets = []
-    for q, g in zip(reqs, gaps):
+    for q, g in zip(arrivals, gaps):
         due += g
         while time.perf_counter() < due:
             pass
@@ -310,8 +312,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--requests", type=int, default=64, help="requests per step")
-    ap.add_argument("--rates", default="0.5,0.8",
+    ap.add_argument("--rates", default="0.3,0.5,0.7,0.85,1.0",
                     help="Poisson arrival rates as fractions of the saturation rate")
+    ap.add_argument("--passes", type=int, default=5,
+                    help="rounds of the step's requests per Poisson rate")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 accuracy sample")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -478,8 +482,9 @@ def main():
 
     latency = []
     for i, frac in enumerate(float(x) for x in args.rates.split(",") if x):
-        thr, p50, p99 = poisson_run(srv, reqs, frac * sat_rate, 100 + i)
+        thr, p50, p99 = poisson_run(srv, reqs, frac * sat_rate, 100 + i, args.passes)
         latency.append({"rate_frac": frac, "rate_req_s": frac * sat_rate,
+                        "requests": len(reqs) * args.passes,
                         "solves_s": thr, "p50_ms": p50 * 1e3, "p99_ms": p99 * 1e3})
     srv.close()
 
@@ -502,18 +507,29 @@ def main():
     offs = torch.cat([q["off"][1:] - q["off"][:-1] for q in reqs])
     off_all = solver.offsets_from_ks(offs)
     idx_all = torch.cat([q["idx"] for q in reqs])
+    # the plain forms through plans: their workspace (and the one-wavefront
+    # form's scratch) allocated once, outside the timed calls
     batch = {}
-    for form in ("plain_wave", "plain", "tuned"):
-        o = solver.solve(G, idx_all, k=256, offsets=off_all, form=form)
+    for form in ("plain_wave", "plain", "tuned", "tuned_plan"):
+        if form == "tuned":
+            o = solver.solve(G, idx_all, k=256, offsets=off_all, form=form)
+            run = lambda: solver.solve(G, idx_all, k=256, offsets=off_all, form="tuned", out=o)
+        else:
+            plan = solver.Plan(G, idx_all, k=256, offsets=off_all,
+                               form="tuned" if form == "tuned_plan" else form)
+            o = plan.execute()
+            run = plan.execute
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for _ in range(2):
-            solver.solve(G, idx_all, k=256, offsets=off_all, form=form, out=o)
+            run()
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t1) * 1e3 / 2
         batch[form] = {"value": tot / (ms / 1e3), "ms": ms}
         if form == "plain_wave":  # the plain form's outputs, for its own accuracy
             plain_out = [t.cpu().numpy() for t in (o[0], o[1], o[3])]
+        if form != "tuned":
+            plan.close()
 
     # accuracy of the served outputs (last timed step) on a 4096-subset sample
     lmin = torch.cat([r[0][0] for r in res]).cpu().numpy()
@@ -602,6 +618,37 @@ def main():
                "sample": f"first {done} subsets of the step's request wave (same k mix), FP64 "
                          f"oracle (full reorth + Sturm bisection), OpenMP over subsets, "
                          f"{spent:.1f} s"}
+        # the reference's own algorithm as it stands: FP64, no
+        # reorthogonalisation, TQLRAT over all eigenvalues, on the same leading
+        # subsets; its extremes against the FP64 oracle's on that sample
+        fd, fspent, ch = 0, 0.0, 64
+        f_lo, f_hi, f_err = [], [], []
+        while fspent < args.cpu_seconds / 2 and fd < tot:
+            hi = min(tot, fd + ch)
+            t2 = time.perf_counter()
+            lo_, hi_, ie_ = oracle.faithful_extremes(
+                Gh, idx_h[off_h[fd]:off_h[hi]], offsets=off_h[fd:hi + 1] - off_h[fd],
+                nthreads=cores)
+            fspent += time.perf_counter() - t2
+            f_lo.append(lo_)
+            f_hi.append(hi_)
+            f_err.append(ie_)
+            fd = hi
+            ch = min(ch * 2, 4096)
+        f_lo, f_hi, f_err = np.concatenate(f_lo), np.concatenate(f_hi), np.concatenate(f_err)
+        nchk = min(fd, 2048)
+        o_lo, o_hi, _, _ = oracle.solve_batch(Gh, idx_h[:off_h[nchk]], offsets=off_h[:nchk + 1],
+                                              nthreads=cores)
+        ferr = np.maximum(np.abs(f_lo[:nchk] - o_lo) / np.abs(o_lo),
+                          np.abs(f_hi[:nchk] - o_hi) / np.abs(o_hi))
+        cpu["reference_faithful"] = {
+            "value": fd / fspent, "unit": "eig-solves/s", "cores": cores,
+            "sample": f"first {fd} subsets of the step's request wave, FP64 Lanczos without "
+                      f"reorthogonalisation + TQLRAT (all eigenvalues), OpenMP over subsets, "
+                      f"{fspent:.1f} s",
+            "ierr_nonzero": int(np.count_nonzero(f_err)),
+            "max_rel_err_vs_oracle": float(np.nanmax(ferr)),
+            "median_rel_err_vs_oracle": float(np.nanmedian(ferr)), "checked": int(nchk)}
         del Gh
         # the small uniform configuration runs whole on the host: Phi 64 x 256
         # (seed 0xC1), 1024 subsets of k = 8 (stream 0xC1+1), one thread and
@@ -616,7 +663,15 @@ def main():
                 oracle.solve_batch(G1, i1, k=8, nthreads=th)
                 reps += 1
             c1[f"solves_s_{th}_threads"] = 1024 * reps / (time.perf_counter() - t2)
-        cpu["c1"] = dict(c1, workload="Phi 64x256, 1024 subsets of k=8, FP64 oracle")
+            oracle.faithful_extremes(G1, i1, k=8, nthreads=th)  # warm
+            reps, t2 = 0, time.perf_counter()
+            while time.perf_counter() - t2 < 0.5:
+                oracle.faithful_extremes(G1, i1, k=8, nthreads=th)
+                reps += 1
+            c1[f"faithful_solves_s_{th}_threads"] = 1024 * reps / (time.perf_counter() - t2)
+        cpu["c1"] = dict(c1, workload="Phi 64x256, 1024 subsets of k=8, FP64 oracle (full "
+                                      "reorth + Sturm) and the reference-faithful path (no "
+                                      "reorth, TQLRAT)")
 
     line = {
         "metric": METRIC,
@@ -647,6 +702,7 @@ def main():
             "nonzero_status": int(np.count_nonzero(plain_out[2]))}),
         "plain_multiwave": batch["plain"],
         "tuned_batch": batch["tuned"],
+        "tuned_plan": batch["tuned_plan"],
         "vs_plain": value / batch["plain_wave"]["value"],
         "vs_plain_multiwave": value / batch["plain"]["value"],
         "served_f32": dict(served_f32, vs_plain_multiwave=served_f32["value"] / batch["plain"]["value"]),

@@ -56,7 +56,8 @@ typedef enum {
 #define PL_FORM_PLAIN_WAVE 2 /* one 64-lane wavefront per subset at every k: beyond
                                 k = 64 the wavefront streams G_S and the basis
                                 from a global scratch (512 KiB per resident
-                                wavefront, allocated per call); FP32 throughout,
+                                wavefront, allocated per call, or once by a plan
+                                of this form); FP32 throughout,
                                 one launch per k; no precision flags            */
 /* precision flags, OR-ed into the form: FP32 or FP64 Gram-Schmidt
    coefficients at every k > 64 (subsets of k <= 64 always use FP32) */

@@ -74,6 +74,7 @@ def _declare(L):
         "or_solve_batch": (None, [P, I64, I64, P, P, I32, I64, P, P, P, P, I32]),
         "or_solve_batch_alleig": (None, [P, I64, I64, P, I32, I64, P, P, P, I32]),
         "or_solve_batch_faithful": (None, [P, I64, P, I32, I64, P, P, P, I32]),
+        "or_faithful_extremes": (None, [P, I64, I64, P, P, I32, I64, P, P, P, I32]),
         "or_upper": (DBL, [P, I32, I32]),
         "or_upper_batch": (None, [P, I64, I64, P, P, I32, I64, I32, P, P, I32]),
     }
@@ -268,3 +269,25 @@ def upper_batch(G, idx, k=None, offsets=None, iters=10, nthreads=1, ld=None):
     lib().or_upper_batch(_p(G), n, ld, _p(ix), offp, kk, nsub, int(iters), _p(up), _p(st),
                          nthreads)
     return up, st
+
+
+def faithful_extremes(G, idx, k=None, offsets=None, nthreads=1, ld=None):
+    """The reference's path as it stands (FP64, no reorthogonalisation,
+    TQLRAT over all eigenvalues) on FP32 G: lambda_min, lambda_max, ierr."""
+    G = _c(G, np.float32)
+    n = G.shape[0]
+    ld = G.shape[1] if ld is None else ld
+    ix = _c(idx, np.int32)
+    if offsets is not None:
+        off = _c(offsets, np.int64)
+        nsub = len(off) - 1
+        offp, kk = _p(off), 0
+    else:
+        nsub = len(ix) // k
+        offp, kk = None, k
+    lo = np.zeros(nsub)
+    hi = np.zeros(nsub)
+    ierr = np.zeros(nsub, np.int32)
+    lib().or_faithful_extremes(_p(G), n, ld, _p(ix), offp, kk, nsub, _p(lo), _p(hi), _p(ierr),
+                               nthreads)
+    return lo, hi, ierr

@@ -600,3 +600,37 @@ void or_upper_batch(const float *G, int64_t n, int64_t ld, const int32_t *idx,
     free(A);
   }
 }
+
+/* Reference-faithful extremes over an FP32 G and CSR index sets: each G_S is
+   gathered into FP64 and run through the reference's path as it stands,
+   ParaLanczos.cu:1285-1374 (no reorthogonalisation) then TQLRAT (:32-189)
+   for every eigenvalue (:1454-1462); lmin/lmax are the ends of the ascending
+   list, ierr TQLRAT's error code. The baseline the GPU path is timed against
+   in its reference form (FP64 throughout, all eigenvalues). */
+void or_faithful_extremes(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                          const int64_t *offsets, int kuni, int64_t nsub, double *lmin,
+                          double *lmax, int32_t *ierr, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+  {
+    double *A = (double *)malloc(sizeof(double) * 256 * 256);
+    double *al = (double *)malloc(sizeof(double) * 256);
+    double *be = (double *)malloc(sizeof(double) * 256);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t s = 0; s < nsub; ++s) {
+      const int64_t o = offsets ? offsets[s] : s * (int64_t)kuni;
+      const int k = offsets ? (int)(offsets[s + 1] - offsets[s]) : kuni;
+      if (k < 1 || k > 256 || or_gather(G, n, ld, idx + o, k, A) != 0) {
+        lmin[s] = lmax[s] = NAN;
+        ierr[s] = -1;
+        continue;
+      }
+      or_lanczos_noreorth(A, k, al, be);
+      for (int i = 0; i < k; ++i) be[i] = be[i] * be[i];
+      ierr[s] = or_tqlrat(k, al, be);
+      lmin[s] = al[0];
+      lmax[s] = al[k - 1];
+    }
+    free(A); free(al); free(be);
+  }
+}

@@ -84,6 +84,11 @@ void or_solve_batch_faithful(const double *G, int64_t n, const int32_t *idx, int
                              int64_t nsub, double *eig, double *det, int32_t *ierr,
                              int nthreads);
 
+/* reference-faithful path (FP64 gather of the FP32 G, no reorth, TQLRAT)
+   over CSR or uniform index sets; the ends of the ascending eigenvalues */
+void or_faithful_extremes(const float *G, int64_t n, int64_t ld, const int32_t *idx,
+                          const int64_t *offsets, int kuni, int64_t nsub, double *lmin,
+                          double *lmax, int32_t *ierr, int nthreads);
 /* power-iteration estimate of lambda_max (UPPER_KER): ones/sqrt(k) start,
    `iters` steps of w = A v, b = ||w||, v = w / b; returns the last b */
 double or_upper(const double *A, int k, int iters);

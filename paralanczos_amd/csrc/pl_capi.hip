@@ -237,11 +237,14 @@ bool job_is_async(const SolveJob &j) {
 }
 
 // one launch of a variant over the job's slots; the one-wavefront variant
-// with a scratch allocated on the stream for the launch
+// on the job's scratch, or on one allocated on the stream for the launch
 hipError_t launch_variant(const Variant &v, const pl::SolveArgs &a, hipStream_t s,
-                          int64_t max_slots) {
+                          int64_t max_slots, const SolveJob &j) {
   if (!v.wave) return pl::launch_solve(v.lanes, v.pad, pl::MODE_TRIDIAG, a, s, max_slots, v.f64);
   const int64_t want = a.bins ? max_slots : a.nslots;
+  if (j.wave_scratch)
+    return pl::launch_wave(a, j.wave_scratch,
+                           std::max<int64_t>(1, std::min<int64_t>(want, j.wave_slots)), s);
   const int64_t slots = std::max<int64_t>(1, std::min<int64_t>(want, pl::kWaveSlots));
   void *scratch = nullptr;
   hipError_t e = hipMallocAsync(&scratch, pl::wave_scratch_bytes(slots), s);
@@ -364,7 +367,7 @@ pl_status run_job(const SolveJob &j, void *ws, hipStream_t s, int stages = 3) {
     // uniform size, or an explicit variant over a mixed batch: one launch
     if (stages & kStageTridiag) {
       const Variant v = j.form == -1 ? j.explicit_v : job_variant(j.form, j.k);
-      e = launch_variant(v, a, s, 0);
+      e = launch_variant(v, a, s, 0, j);
       if (e != hipSuccess) return hip_fail(e, "solve launch");
     }
     return (stages & kStageBisect) ? run_bisect(j, w, nullptr, s) : PL_OK;
@@ -390,7 +393,7 @@ pl_status run_job(const SolveJob &j, void *ws, hipStream_t s, int stages = 3) {
       if (cnt == 0) continue;
       a.order = w.order + tab[pl::BIN_START + b];
       a.nslots = cnt;
-      e = launch_variant(bp.var[b], a, s, 0);
+      e = launch_variant(bp.var[b], a, s, 0, j);
       if (e != hipSuccess) return hip_fail(e, "per-k solve launch");
     }
   }
@@ -612,6 +615,7 @@ pl_status pl_solve_variant(const float *G, int64_t n, int64_t ld, const int32_t 
 struct pl_plan {
   SolveJob job;
   void *ws = nullptr;
+  void *wave = nullptr;  // the one-wavefront form's scratch
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -631,9 +635,17 @@ pl_status pl_plan_create(pl_plan **plan, const float *G, int64_t n, int64_t ld,
   pl_plan *p = new pl_plan;
   p->job = j;
   hipError_t e = hipMalloc(&p->ws, pl::workspace_bytes(nsub > 0 ? nsub : 1, k));
+  if (e == hipSuccess && (form & kFormMask) == PL_FORM_PLAIN_WAVE && k > 64) {
+    // resident wavefronts of the one-wavefront form: its slots, at most kWaveSlots
+    const int64_t slots = std::max<int64_t>(1, std::min<int64_t>(nsub, pl::kWaveSlots));
+    e = hipMalloc(&p->wave, pl::wave_scratch_bytes(slots));
+    p->job.wave_scratch = (float *)p->wave;
+    p->job.wave_slots = slots;
+  }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking);
   if (e != hipSuccess) {
     if (p->ws) (void)hipFree(p->ws);
+    if (p->wave) (void)hipFree(p->wave);
     delete p;
     return hip_fail(e, "plan allocation");
   }
@@ -684,6 +696,7 @@ pl_status pl_plan_destroy(pl_plan *p) {
   // stream: wait for the device before releasing it
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (p->ws) (void)hipFree(p->ws);
+  if (p->wave) (void)hipFree(p->wave);
   delete p;
   if (e != hipSuccess) return hip_fail(e, "plan teardown");
   return PL_OK;

@@ -155,6 +155,10 @@ struct SolveJob {
   // tuned mixed batches: the bins other than the widest on a helper stream,
   // so that they fill the wide kernel's tail
   bool overlap_bins = true;
+  // one-wavefront plain form: a scratch of wave_scratch_bytes(wave_slots)
+  // owned by the caller (a plan); nullptr = allocated on the stream per launch
+  float *wave_scratch = nullptr;
+  int64_t wave_slots = 0;
 };
 
 // the stream-ordered launches of one solve on a workspace of

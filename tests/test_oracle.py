@@ -341,3 +341,33 @@ def test_upper_planted_mistakes_are_caught():
     assert abs(planted(np.diag(d), 10, extra=1) - expect) > 1e-6
     assert abs(planted(np.diag(d), 10, extra=-1) - expect) > 1e-6
     assert abs(planted(np.diag(d), 10, square=True) - expect) > 1e-3
+
+
+def test_faithful_extremes_driver():
+    """The FP32-G, CSR form of the reference-faithful path equals the FP64-G
+    driver (pinned on the example matrix above) on the same data, and on
+    well-conditioned Gram subsets its extremes agree with dsyevd."""
+    rng = np.random.default_rng(21)
+    X = rng.normal(size=(96, 150)).astype(np.float32)
+    X /= np.linalg.norm(X, axis=0)
+    G = (X.T @ X).astype(np.float32)
+    G = ((G + G.T) * np.float32(0.5)).astype(np.float32)
+    idx = oracle.gen_subsets_uniform(6, 30, 10, 150)
+    lo, hi, ierr = oracle.faithful_extremes(G, idx, k=10, nthreads=2)
+    eig, _, ierr2 = oracle.solve_batch_faithful(G.astype(np.float64), idx, 10)
+    assert np.all(ierr == 0) and np.all(ierr2 == 0)
+    e = eig.reshape(30, 10)
+    assert np.array_equal(lo, e[:, 0]) and np.array_equal(hi, e[:, -1])
+    ks = np.array([4, 16, 1, 9], np.int64)
+    off = np.concatenate([[0], np.cumsum(ks)])
+    idx2 = oracle.gen_subsets_csr(9, off, 150)
+    lo2, hi2, ierr3 = oracle.faithful_extremes(G, idx2, offsets=off)
+    assert np.all(ierr3 == 0)
+    for s in range(4):
+        S = idx2[off[s]:off[s + 1]]
+        w = np.linalg.eigvalsh(G[np.ix_(S, S)].astype(np.float64))
+        assert abs(lo2[s] - w[0]) < 1e-8 and abs(hi2[s] - w[-1]) < 1e-8
+    bad = idx2.copy()
+    bad[0] = 1000
+    lo3, _, ierr4 = oracle.faithful_extremes(G, bad, offsets=off)
+    assert ierr4[0] == -1 and np.isnan(lo3[0]) and ierr4[1] == 0
