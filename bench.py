@@ -306,6 +306,50 @@ def gram_leg(solver, torch, phi, G):
             "library_gemm_ms": lib_ms, "vs_library": lib_ms / ms, "max_abs_diff_vs_library": diff}
 
 
+def kernel_times(solver, torch, dev):
+    """Stage 1 of the k = 128 and k = 256 kernels on the serving G: 6200
+    subsets each (one served wave's worth), FP64 and FP32 Gram-Schmidt
+    coefficients, 3 timed launches after a warm one (HIP events on the
+    launching stream); microseconds per subset and CU; the eigenvalues of the
+    first 48 subsets of each against the FP64 oracle."""
+    import oracle
+
+    G = solver.gram(solver.gen_phi("dct", SEED_PHI, M, N))
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    stream = torch.cuda.current_stream()
+    out = {"workload": "serving G (DCT-II 4096 x 32768), 6200 seeded subsets per k", "cus": cus}
+    Gh = None
+    for k in (128, 256):
+        nsub = 6200
+        idx = solver.gen_subsets(SEED_SAMPLE + k, nsub, k, N)
+        ws = torch.empty(solver.workspace_size(nsub, k), dtype=torch.uint8, device=dev)
+        for reorth in ("f64", "f32"):
+            o = solver.solve_stage(G, idx, 1, ws, k=k, reorth=reorth)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(3):
+                solver.solve_stage(G, idx, 1, ws, k=k, reorth=reorth, out=o)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 3
+            lo, hi, _, st = solver.solve(G, idx[:48 * k].contiguous(), k=k, reorth=reorth)
+            if Gh is None:
+                Gh = G.cpu().numpy()
+            olo, ohi, _, ost = oracle.solve_batch(Gh, idx[:48 * k].cpu().numpy(), k=k,
+                                                  nthreads=16)
+            err = max(float(np.max(np.abs(lo.cpu().numpy() - olo) / olo)),
+                      float(np.max(np.abs(hi.cpu().numpy() - ohi) / ohi)))
+            out[f"k{k}_{reorth}"] = {
+                "ms": ms, "us_per_subset_cu": ms * 1e3 * cus / nsub,
+                "tflops": nsub * flops_per_solve(k) / (ms / 1e3) / 1e12,
+                "frac": nsub * flops_per_solve(k) / (ms / 1e3) / 1e12 / FP32_PEAK_TFLOPS,
+                "max_rel_err_48": err, "nonzero_status": int(np.count_nonzero(st.cpu().numpy()))}
+            print(json.dumps({f"k{k}_{reorth}": out[f"k{k}_{reorth}"]}), flush=True)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,6 +378,10 @@ def main():
                     help="only run the gather stage over the dominant subsets (rocprof counters)")
     ap.add_argument("--profile-gram", action="store_true",
                     help="run pl_gram on the serving frame 1 + 3 times and exit (for rocprofv3)")
+    ap.add_argument("--kernel-times", action="store_true",
+                    help="time stage 1 (gather + Lanczos) of the k = 128 and k = 256 kernels in "
+                         "both coefficient precisions on the serving G, check a sample against "
+                         "the oracle, print one JSON line and exit")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy this many MiB device to device and exit (calibrates the "
                          "profiler's byte counters against a transfer of known size)")
@@ -372,6 +420,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"profile_c2": True, "subsets": 1 << 20, "k": 32}), flush=True)
         return 0
+    if args.kernel_times:
+        return kernel_times(solver, torch, dev)
     t_setup = time.perf_counter()
     phi = solver.gen_phi("dct", SEED_PHI, M, N)
     if args.profile_gram:

@@ -983,8 +983,11 @@ __global__ __launch_bounds__(WIDE) void solve_wide_kernel(SolveArgs a) {
 // part h = t / KC) holds G_S[i][KC/Q h .. KC/Q (h + 1) - 1] in VGPRs. The
 // Lanczos basis V keeps columns 0..L-1 in LDS (row-major, stride S). At
 // KC = 256 an FP32 G_S and an FP32 basis do not both fit one CU's LDS, and at
-// KC = 128 three subsets share a CU, so columns L..KC-1 live in VGPRs, column
-// L + Q m + h of row i as vr[m] on lane (i, h).
+// KC = 128 three subsets share a CU, so columns L..KC-1 live in VGPRs: the
+// columns L + Q m + h (m = 0, 1, ...) of part h, row i, on lane (i, h), as a
+// shift register: the newest in slot vr[0], column m in slot nm_h - 1 - m
+// once nm_h of them are formed. Adding a column shifts the part's slots up by
+// one (a wave-uniform branch: the part index is uniform per wavefront).
 // Gram-Schmidt coefficients of the LDS columns are column dot products (the
 // lanes split into columns x row groups); those of the register columns come
 // from a wavefront reduce-scatter of the lanes' products (wave_rs) and a sum
@@ -1041,6 +1044,8 @@ struct HRShared {
     float lh[KC];    // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
   };
   T lwT[sizeof(T) == 8 ? KC : 4];  // w in T (FP64 coefficients only): one conversion per entry
+  // the register columns' coefficients per part in slot order (16-byte reads)
+  alignas(16) float lhr[Q][C::R > 0 ? (C::R + 3) / 4 * 4 : 4];
   union {
     float pw[Q][KC];    // matvec partials of the row parts
     float part[Q][KC];  // Gram-Schmidt update partials of the row parts
@@ -1404,11 +1409,15 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
       for (int g = 1; g < ng; ++g) hc += sh.cp[g * ncl + i];
     } else if constexpr (C::R > 0) {
       if (i >= C::L && i <= j) {
+        // column i is the m-th register column of part hh, in slot
+        // nm_hh - 1 - m with nm_hh = (j - L - hh) / Q + 1 formed in that part
         const int m = (i - C::L) / Q, hh = (i - C::L) % Q;
-        T s = sh.rpart[hh][0][m];
+        const int sl = (j - C::L - hh) / Q - m;
+        T s = sh.rpart[hh][0][sl];
 #pragma unroll
-        for (int u = 1; u < C::WPH; ++u) s += sh.rpart[hh][u][m];
+        for (int u = 1; u < C::WPH; ++u) s += sh.rpart[hh][u][sl];
         hc = s;
+        sh.lhr[hh][sl] = (float)hc;
       }
     }
     sh.lh[i] = (float)hc;
@@ -1449,15 +1458,19 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     }
   }
   if constexpr (C::R > 0) {
-    if (nm > 0) {
+    // slots 0..nm_h-1 of this part, coefficients read 4 at a time in slot
+    // order (a broadcast: the part is uniform per wavefront)
+    const int hu = __builtin_amdgcn_readfirstlane(h);
+    const int nmh = j >= C::L + hu ? (j - C::L - hu) / Q + 1 : 0;
+    const float4 *lr = reinterpret_cast<const float4 *>(sh.lhr[hu]);
 #pragma unroll
-      for (int m0 = 0; m0 < C::R; m0 += 8) {
-        if (m0 >= nm) break;
-#pragma unroll
-        for (int m = m0; m < m0 + 8 && m < C::R; m += 2) {
-          u0 = __builtin_fmaf(vr[m], sh.lh[C::L + Q * m + h], u0);
-          if (m + 1 < C::R) u1 = __builtin_fmaf(vr[m + 1], sh.lh[C::L + Q * (m + 1) + h], u1);
-        }
+    for (int m0 = 0; m0 < C::R; m0 += 4) {
+      if (m0 < nmh) {  // uniform; slots past nm_h hold zeros
+        const float4 cf = lr[m0 / 4];
+        u0 = __builtin_fmaf(vr[m0], cf.x, u0);
+        if (m0 + 1 < C::R) u1 = __builtin_fmaf(vr[m0 + 1], cf.y, u1);
+        if (m0 + 2 < C::R) u2 = __builtin_fmaf(vr[m0 + 2], cf.z, u2);
+        if (m0 + 3 < C::R) u3 = __builtin_fmaf(vr[m0 + 3], cf.w, u3);
       }
     }
   }
@@ -1570,6 +1583,11 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
 #pragma unroll
       for (int m = 0; m < C::RA; ++m) vr[m] = 0.f;
       for (int e = t; e < KC * C::S; e += C::NT) VL[e] = 0.f;
+      // register-column coefficients: the update reads whole groups of 4
+      // slots, so slots not yet formed must hold zeros, not stale LDS
+      if constexpr (C::R > 0)
+        for (int e = t; e < (int)(sizeof(sh.lhr) / sizeof(float)); e += C::NT)
+          (&sh.lhr[0][0])[e] = 0.f;
       float scale = 1.0f / sqrtf((float)k);
       float vi = (i < k) ? scale : 0.f;
       __syncthreads();
@@ -1657,20 +1675,17 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
           scale = 1.0f / sqrtf(hr_beta2(w, sh));
         }
         vi = w * scale;
-        // basis column j + 1: LDS, or register slot (j + 1 - L) / Q of one part
+        // basis column j + 1: LDS, or slot 0 of the register shift of its
+        // part (the part is uniform per wavefront)
         const int c = j + 1;
-        if (c < C::L) {
-          if (h == 0) VL[i * C::S + c] = vi;
-        } else if constexpr (C::R > 0) {
-          const int m = (c - C::L) / Q;
-          const bool mine = ((c - C::L) % Q) == h;
-          // selects within the group of 8 slots holding m only (uniform branch)
+        if (c < C::L && h == 0) VL[i * C::S + c] = vi;
+        if constexpr (C::R > 0) {
+          const int hu = __builtin_amdgcn_readfirstlane(h);
+          const int cc = c - C::L;
+          if (cc >= 0 && cc % Q == hu) {
 #pragma unroll
-          for (int r8 = 0; r8 < C::R; r8 += 8) {
-            if ((m >> 3) == (r8 >> 3)) {
-#pragma unroll
-              for (int r = r8; r < r8 + 8 && r < C::R; ++r) vr[r] = (mine && r == m) ? vi : vr[r];
-            }
+            for (int r = C::R - 1; r > 0; --r) vr[r] = vr[r - 1];
+            vr[0] = vi;
           }
         }
         my_b = (i == j + 1) ? bj : my_b;
