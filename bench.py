@@ -612,12 +612,16 @@ def main():
     F = n256 * flops_per_solve(DOMINANT_K)
     tflops = F / (dom_ms / 1e3) / 1e12
     gbs = B / (dom_ms / 1e3) / 1e9
-    traffic, lds = None, None
+    traffic, lds, traffic_split = None, None, None
     prof = os.path.join(ROOT, "profiles", "solve_k256_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
             tj = json.load(f)
         traffic = tj.get("traffic_bytes_per_launch")
+        traffic_split = {"hbm_read": tj.get("hbm_read_bytes_per_launch"),
+                         "infinity_cache_read": tj.get("infinity_cache_read_bytes_per_launch"),
+                         "dram_destined_read": tj.get("dram_destined_read_bytes_per_launch"),
+                         "write": tj.get("write_bytes_per_launch")}
         # LDS bytes the lanes request per launch (rocprof SQ_INSTS_LDS_*_BANDWIDTH
         # of the same launch), over this run's launch time
         if tj.get("lds_load_bytes_per_launch"):
@@ -635,10 +639,16 @@ def main():
     gprof = os.path.join(ROOT, "profiles", "gather_k256_traffic.json")
     if os.path.exists(gprof):
         with open(gprof) as f:
-            fb = json.load(f).get("fetch_bytes_per_launch")
-        if fb:
-            gather["hbm_fetch_GBs"] = fb / (gather_ms / 1e3) / 1e9
-            gather["hbm_frac"] = gather["hbm_fetch_GBs"] / HBM_PEAK_GBS
+            gj = json.load(f)
+        # DRAM-destined line fills (TCC_EA0_RDREQ_DRAM x 128 B), of which the
+        # Infinity Cache serves the modelled share; hbm_frac uses the rest
+        db, hb = gj.get("dram_destined_bytes_per_launch"), gj.get("hbm_read_bytes_per_launch")
+        if db and hb:
+            gather["dram_destined_GBs"] = db / (gather_ms / 1e3) / 1e9
+            gather["infinity_cache_frac_model"] = gj.get("infinity_cache_hit_fraction_model")
+            gather["hbm_GBs"] = hb / (gather_ms / 1e3) / 1e9
+            gather["hbm_frac"] = gather["hbm_GBs"] / HBM_PEAK_GBS
+            gather["copy_1gib_read_plus_write_GBs"] = gj.get("copy_1gib_read_plus_write_tbs", 0) * 1e3
 
     legs = {}
     if not args.no_legs:
@@ -761,6 +771,7 @@ def main():
                      "nonzero_status": int(np.count_nonzero(status))},
         "roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS, "traffic": traffic,
+                     "traffic_split": traffic_split,
                      "kernel": "solve_rowpart_kernel<256, 2, TRIDIAG, double>",
                      "launch_ms": dom_ms, "subsets_per_launch": n256},
         "roofline_hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
