@@ -160,7 +160,21 @@ def batching_leg(solver, torch, autotune=True):
         res["autotuned"] = {k: list(v[:2]) for k, v in tune.items()}
         res["autotuned_ms"] = ms(lambda: solver.solve(G, idx, k=128, offsets=off, out=out))
         solver.tuning_reset()
-    best = min(res["tuned_ms"], res["tuned_graph_ms"], res.get("autotuned_ms", 1e30))
+        # bin edges tuned too: every variant that takes a size competes at it
+        t0 = time.perf_counter()
+        win, table = solver.autotune_edges(G, list(range(8, 129, 8)), sample=32768)
+        res["autotune_edges_s"] = time.perf_counter() - t0
+        fam = lambda lanes: "lane groups" if lanes < 128 else f"{lanes} lanes"
+        res["edges_sweep_ms"] = {k: {f"{l}/{p}": round(t, 4) for (l, p), t in sorted(v.items())}
+                                 for k, v in table.items()}
+        res["edges_winners"] = {k: list(v[:2]) for k, v in win.items()}
+        ks_sorted = sorted(win)
+        res["edges"] = [{"from_k": k, "family": fam(win[k][0])} for i, k in enumerate(ks_sorted)
+                        if i == 0 or fam(win[k][0]) != fam(win[ks_sorted[i - 1]][0])]
+        res["autotuned_edges_ms"] = ms(lambda: solver.solve(G, idx, k=128, offsets=off, out=out))
+        solver.tuning_reset()
+    best = min(res["tuned_ms"], res["tuned_graph_ms"], res.get("autotuned_ms", 1e30),
+               res.get("autotuned_edges_ms", 1e30))
     res["solves_s_tuned_best"] = nsub / (best / 1e3)
     res["solves_s_plain"] = nsub / (res["plain_ms"] / 1e3)
     res["vs_plain"] = res["plain_ms"] / best

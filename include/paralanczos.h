@@ -187,6 +187,20 @@ pl_status pl_plan_destroy(pl_plan *plan);
  */
 pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klist, int nk,
                       int64_t sample, uint64_t seed, double *best_ms, void *stream);
+/* pl_autotune with the bin edges tuned too: at each listed size every variant
+   that takes it competes, across kernel families, and every unlisted size
+   takes the winner of the next listed size above it, so the boundaries
+   between the families (the bin edges of mixed batches) move to the measured
+   optimum. Different families round differently (all within the stated
+   tolerance); while the table stays fixed, every subset's result is the same
+   alone and in any mix. sweep_ms (nk x PL_TUNE_VARIANTS, may be NULL): the
+   time of variant v = 3 * (index of its lanes in 8, 16, 32, 64, 128, 256,
+   512) + (index of its pad in 0, 1, 4) for each listed size, -1 where that
+   variant does not exist or does not take the size. */
+#define PL_TUNE_VARIANTS 21
+pl_status pl_autotune_edges(const float *G, int64_t n, int64_t ld, const int32_t *klist, int nk,
+                            int64_t sample, uint64_t seed, double *best_ms, double *sweep_ms,
+                            void *stream);
 pl_status pl_tuning_get(int k, int *lanes, int *pad);
 pl_status pl_tuning_set(int k, int lanes, int pad);
 pl_status pl_tuning_reset(void);

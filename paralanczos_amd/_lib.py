@@ -37,6 +37,7 @@ SIGNATURES = {
     "pl_plan_is_graph": (INT, [P]),
     "pl_plan_destroy": (ST, [P]),
     "pl_autotune": (ST, [P, I64, I64, P, INT, I64, U64, P, P]),
+    "pl_autotune_edges": (ST, [P, I64, I64, P, INT, I64, U64, P, P, P]),
     "pl_tuning_get": (ST, [INT, P, P]),
     "pl_tuning_set": (ST, [INT, INT, INT]),
     "pl_tuning_reset": (ST, []),

@@ -11,6 +11,7 @@
 #include <array>
 #include <atomic>
 #include <functional>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -730,8 +731,18 @@ pl_status pl_tuning_reset(void) {
   return PL_OK;
 }
 
-pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klist, int nk,
-                      int64_t sample, uint64_t seed, double *best_ms, void *stream) {
+}  // extern "C"
+
+namespace {
+// The autotuner. Per listed k it times the candidate variants on `sample`
+// seeded subsets of G (three runs, the first a warm-up, the best of the other
+// two) and keeps the fastest. Candidates: the variants of the family of k's
+// default, or with `edges` every variant that takes k, so that the family
+// boundaries (the bin edges of mixed batches) move to the measured optimum.
+// sweep (nk x PL_TUNE_VARIANTS, may be null) gets every candidate's time.
+pl_status autotune(const float *G, int64_t n, int64_t ld, const int32_t *klist, int nk,
+                   int64_t sample, uint64_t seed, bool edges, double *best_ms, double *sweep,
+                   void *stream) {
   if (!G || !klist || nk < 1) return fail(PL_ERR_ARG, "G and klist must be non-null, nk >= 1");
   if (sample < 1 || sample > (1 << 24)) return fail(PL_ERR_ARG, "sample outside [1, 2^24]");
   int kmax = 0;
@@ -754,6 +765,9 @@ pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klis
   pl_status st = e == hipSuccess ? PL_OK : hip_fail(e, "autotune allocation");
   static const int kLanes[] = {8, 16, 32, 64, 128, 256, 512};
   static const int kPads[] = {0, 1, 4};
+  static_assert(sizeof(kLanes) / sizeof(int) * 3 == PL_TUNE_VARIANTS, "variant grid");
+  if (sweep)
+    for (int v = 0; v < nk * PL_TUNE_VARIANTS; ++v) sweep[v] = -1.0;
   for (int i = 0; i < nk && st == PL_OK; ++i) {
     const int k = klist[i];
     e = pl::launch_gen_subsets(seed, 0, sample, nullptr, k, (int)n, idx, s);
@@ -766,9 +780,10 @@ pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klis
     for (int lanes : kLanes) {
       // the lane groups of 8..64 lanes compute bit-identical results, as do
       // the 128-lane paddings; the other kernels each have their own
-      // summation order, so only variants of the family that k's default
-      // belongs to are offered and tuning never changes a result
-      if (pl::variant_capacity(lanes) < k || family(lanes) != family(default_variant(k).lanes))
+      // summation order, so without `edges` only variants of the family that
+      // k's default belongs to are offered and tuning never changes a result
+      if (pl::variant_capacity(lanes) < k ||
+          (!edges && family(lanes) != family(default_variant(k).lanes)))
         continue;
       for (int pad : kPads) {
         if (!pl::solve_variant_exists(lanes, pad)) continue;
@@ -788,6 +803,11 @@ pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klis
           if (rep == 1 || (rep > 1 && ms < t)) t = ms;
         }
         if (st != PL_OK) break;
+        if (sweep) {
+          const int li = (int)(std::find(std::begin(kLanes), std::end(kLanes), lanes) - kLanes);
+          const int pi = (int)(std::find(std::begin(kPads), std::end(kPads), pad) - kPads);
+          sweep[i * PL_TUNE_VARIANTS + 3 * li + pi] = t;
+        }
         if (best.lanes == 0 || t < best_t) {
           best = Variant{lanes, pad, false};
           best_t = t;
@@ -805,8 +825,9 @@ pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klis
   }
   if (st == PL_OK) {
     // bin edges: a size between two measured ones takes the winner of the
-    // next measured size above it (whose lane count covers it) when that
-    // winner belongs to the size's own kernel family, else keeps its default
+    // next measured size above it (whose lane count covers it); without
+    // `edges` only when that winner belongs to the size's own kernel family,
+    // else it keeps its default
     std::vector<int> ks(klist, klist + nk);
     std::sort(ks.begin(), ks.end());
     ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
@@ -816,7 +837,8 @@ pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klis
       while (ks[q] < k) ++q;
       if (ks[q] == k) continue;
       const Variant w = g_tune[ks[q]];
-      g_tune[k] = family(w.lanes) == family(default_variant(k).lanes) ? w : default_variant(k);
+      g_tune[k] = (edges || family(w.lanes) == family(default_variant(k).lanes)) ? w
+                                                                                 : default_variant(k);
     }
   }
   if (e0) (void)hipEventDestroy(e0);
@@ -825,6 +847,21 @@ pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klis
   if (out) (void)hipFree(out);
   if (idx) (void)hipFree(idx);
   return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+pl_status pl_autotune(const float *G, int64_t n, int64_t ld, const int32_t *klist, int nk,
+                      int64_t sample, uint64_t seed, double *best_ms, void *stream) {
+  return autotune(G, n, ld, klist, nk, sample, seed, false, best_ms, nullptr, stream);
+}
+
+pl_status pl_autotune_edges(const float *G, int64_t n, int64_t ld, const int32_t *klist, int nk,
+                            int64_t sample, uint64_t seed, double *best_ms, double *sweep_ms,
+                            void *stream) {
+  return autotune(G, n, ld, klist, nk, sample, seed, true, best_ms, sweep_ms, stream);
 }
 
 pl_status pl_solve_alleig(const float *G, int64_t n, int64_t ld, const int32_t *idx, int k,

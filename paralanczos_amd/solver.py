@@ -342,6 +342,34 @@ def autotune(G, klist, sample=65536, seed=0x7A, stream=True):
     return out
 
 
+TUNE_LANES = (8, 16, 32, 64, 128, 256, 512)
+TUNE_PADS = (0, 1, 4)
+
+
+def autotune_edges(G, klist, sample=65536, seed=0x7A):
+    """pl_autotune_edges: every variant that takes a listed size competes
+    (across kernel families), unlisted sizes take the next listed size's
+    winner. Returns ({k: (lanes, pad, ms)}, {k: {(lanes, pad): ms}})."""
+    L = load()
+    _dev(G, torch.float32, "G")
+    ks = [int(x) for x in klist]
+    kl = (ctypes.c_int32 * len(ks))(*ks)
+    best = (ctypes.c_double * len(ks))()
+    nv = len(TUNE_LANES) * len(TUNE_PADS)
+    sweep = (ctypes.c_double * (len(ks) * nv))()
+    check(L.pl_autotune_edges(_p(G), G.shape[0], G.stride(0), ctypes.cast(kl, ctypes.c_void_p),
+                              len(ks), int(sample), int(seed), ctypes.cast(best, ctypes.c_void_p),
+                              ctypes.cast(sweep, ctypes.c_void_p), _stream()),
+          "pl_autotune_edges")
+    win, table = {}, {}
+    for i, k in enumerate(ks):
+        lanes, pad = tuning_get(k)
+        win[k] = (lanes, pad, float(best[i]))
+        table[k] = {(TUNE_LANES[v // 3], TUNE_PADS[v % 3]): float(sweep[i * nv + v])
+                    for v in range(nv) if sweep[i * nv + v] >= 0}
+    return win, table
+
+
 def tuning_get(k):
     lanes, pad = ctypes.c_int(), ctypes.c_int()
     check(load().pl_tuning_get(int(k), ctypes.byref(lanes), ctypes.byref(pad)), "pl_tuning_get")

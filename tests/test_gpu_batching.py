@@ -159,3 +159,41 @@ def test_autotune(mixed):
     finally:
         solver.tuning_reset()
     assert solver.tuning_get(24) == (32, 4)
+
+
+def test_autotune_edges(mixed):
+    """Tuned bin edges: at each listed size every variant that takes it
+    competes across kernel families; unlisted sizes take the next listed
+    size's winner. With the table fixed, the binned solve still matches the
+    oracle, and each subset's result is the same alone and in the mix."""
+    G, ks, off, idx = mixed
+    klist = [8, 24, 40, 64, 72, 128]
+    try:
+        win, table = solver.autotune_edges(G, klist, sample=4096)
+        for k in klist:
+            lanes, pad, ms = win[k]
+            assert ms > 0 and (lanes, pad) in table[k] and table[k][(lanes, pad)] == ms
+            assert min(table[k].values()) == ms
+            # every variant that takes k was timed, none that cannot
+            for (l, p) in table[k]:
+                assert (l if l < 256 else l // 2) >= k
+            cands = {(l, p) for l in (8, 16, 32, 64) if l >= k for p in (0, 1, 4)}
+            cands |= {(128, p) for p in (1, 4) if k <= 128} | {(256, 4)} | {(512, 4)}
+            assert set(table[k]) == {c for c in cands if (c[0] if c[0] < 256 else c[0] // 2) >= k}
+        for k in range(1, 129):
+            nxt = min(x for x in klist if x >= k)
+            assert solver.tuning_get(k) == solver.tuning_get(nxt), k
+        out = _np(solver.solve(G, idx, k=128, offsets=off))
+        assert not out[3].any()
+        o = off.cpu().numpy()
+        ih = idx.cpu().numpy()
+        olo, ohi, _, _ = oracle.solve_batch(G.cpu().numpy(), ih, offsets=o, nthreads=8)
+        assert rel(out[0], olo).max() <= RTOL and rel(out[1], ohi).max() <= RTOL
+        # alone: each of 40 subsets solved by itself with the same table
+        for s in range(0, 3000, 75):
+            k = int(o[s + 1] - o[s])
+            one = _np(solver.solve(G, idx[o[s]:o[s + 1]].contiguous(), k=k))
+            assert one[0][0] == out[0][s] and one[1][0] == out[1][s], s
+    finally:
+        solver.tuning_reset()
+    assert solver.tuning_get(24) == (32, 4) and solver.tuning_get(100) == (256, 4)
