@@ -1201,7 +1201,7 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
 template <int KC, int Q, typename T>
 __device__ __forceinline__ float hr_beta2(float w, HRShared<KC, Q, T> &sh) {
   int t = threadIdx.x;
-  if constexpr (KC == 256) asm volatile("" : "+v"(t));
+  asm volatile("" : "+v"(t));
   const int i = t & (KC - 1);
   const double sd = wave_sum_f64((double)w * (double)w);
   if (t < KC) {
@@ -1316,7 +1316,7 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   constexpr int SH = NC == 16 ? 2 : 3;
   // lane coordinates derived here rather than kept live across the step
   int t = threadIdx.x;
-  if constexpr (KC == 256) asm volatile("" : "+v"(t));
+  asm volatile("" : "+v"(t));
   const int i = t & (KC - 1), h = t >> C::LOGK, wv = (t >> 6) & (C::WPH - 1), q = t & 63;
   float ws = 0.f;
   if constexpr (WANT_PRE) ws = g_sum_dpp<64>(w * w);
@@ -1600,7 +1600,7 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
       for (int j = 0; j < k; ++j) {
         // lane coordinates derived here rather than kept live across the step
         int tl = threadIdx.x;
-        if constexpr (KC == 256) asm volatile("" : "+v"(tl));
+        asm volatile("" : "+v"(tl));
         const int il = tl & (KC - 1), hl = tl >> C::LOGK;
         // matvec partial over this part's columns; alpha's partial over it:
         // exact products v_i mv_i summed in FP64 (the parts' partials sum to
