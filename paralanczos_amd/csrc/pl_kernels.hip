@@ -1512,39 +1512,57 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
   if (h == 0) lidx[i] = my;
   __syncthreads();
 
-  // ---- (a) gather: lane (i, h) loads G_S[i][HW h + c] directly. The
-  // entries (i, c) and (c, i) are the same word of G; load v of every lane
-  // fetches column HW h + ((i mod HW) xor v), so the two lanes that need a
-  // word issue it with the same instruction and the second request meets
-  // the first line in the L2 instead of HBM. A butterfly of selects (bit b
-  // of i mod HW swaps the slots v and v xor 2^b) then puts column HW h + c
-  // into g[c]. ----
+  // ---- (a) gather: the upper triangle of G_S (r <= c, KC (KC + 1) / 2
+  // words) is fetched once, each word G[min(idx_r, idx_c)][max(..)] by one
+  // lane: lane t takes the entries t + NT m of the triangle folded into a
+  // KC/2 x (KC + 1) rectangle (row pr holds row pr of the triangle, then row
+  // KC - 1 - pr), so every lane has about KC (KC + 1) / (2 NT) independent
+  // loads in flight and each word of G is requested once. The words go to
+  // LDS in the basis area (free until the recurrence starts), rectangle row
+  // stride TS = KC + 3, and each lane then reads its half row from there:
+  // G_S[i][c] is triangle entry (min(i, c), max(i, c)). ----
   float g[C::HW];
   bool fin = true;
   {
-    const int im = i & (C::HW - 1);
+    constexpr int TS = KC + 3, NP = (KC / 2) * (KC + 1), MI = (NP + C::NT - 1) / C::NT;
+    static_assert((KC / 2) * TS <= KC * C::S, "folded triangle fits the basis area");
+    float x[MI];
 #pragma unroll
-    for (int v = 0; v < C::HW; ++v) {
-      const int col = C::HW * h + (im ^ v);
-      float x = 0.f;
-      if (st == 0 && i < k && col < k) {
-        const int ic = lidx[col];
-        const int64_t lo = min(ic, my), hi = max(ic, my);
-        x = a.G[lo * a.ld + hi];
+    for (int m = 0; m < MI; ++m) {
+      const int p = t + C::NT * m;
+      const int pr = p / (KC + 1), qc = p - pr * (KC + 1);
+      const bool first = qc < KC - pr;
+      const int r = first ? pr : KC - 1 - pr, c = first ? pr + qc : qc - 1;
+      float y = 0.f;
+      if (st == 0 && p < NP && c < k) {  // r <= c
+        const int ir = lidx[r], ic = lidx[c];
+        const int64_t lo = min(ir, ic), hi = max(ir, ic);
+        y = a.G[lo * a.ld + hi];
       }
-      g[v] = x;
-      fin = fin && isfinite(x);
+      x[m] = y;
     }
 #pragma unroll
-    for (int b = 1; b < C::HW; b <<= 1) {
-      const bool sw = (im & b) != 0;
-#pragma unroll
-      for (int v = 0; v < C::HW; ++v) {
-        if (v & b) continue;
-        const float lo = g[v], hi = g[v | b];
-        g[v] = sw ? hi : lo;
-        g[v | b] = sw ? lo : hi;
+    for (int m = 0; m < MI; ++m) {
+      const int p = t + C::NT * m;
+      if (p < NP) {
+        const int pr = p / (KC + 1), qc = p - pr * (KC + 1);
+        VL[pr * TS + qc] = x[m];
       }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < C::HW; ++v) {
+      const int c = C::HW * h + v;
+      float y = 0.f;
+      if (st == 0 && i < k && c < k) {
+        const int r0 = min(i, c), c0 = max(i, c);
+        const bool first = r0 < KC / 2;
+        const int pr = first ? r0 : KC - 1 - r0;
+        const int qc = first ? c0 - r0 : c0 + 1;
+        y = VL[pr * TS + qc];
+      }
+      g[v] = y;
+      fin = fin && isfinite(y);
     }
   }
   if (hr_any(!fin, sh)) st |= ST_NONFINITE;
