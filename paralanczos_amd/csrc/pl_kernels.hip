@@ -1053,7 +1053,7 @@ struct HRShared {
   };
   union {
     float lvn[KC];      // next Lanczos vector, unnormalised: the matvec input
-    T cp[C::NT];        // LDS-column dot partials: row group g, column c at g * ncl + c
+    T cp[(C::L + 3) / 4 * 4];  // LDS-column coefficients, written by the column groups
     double redd[16];    // bounds reduction, after the recurrence
   };
   double ra[C::NW];     // alpha partials (FP64) of all wavefronts
@@ -1229,80 +1229,6 @@ __device__ __forceinline__ bool hr_any(bool p, HRShared<KC, Q, T> &sh) {
   return any != 0;
 }
 
-// Split of the column dots over the lanes, per number of LDS columns ncl
-// (1..L): ng = min(16, NT / ncl) row groups of RR rows (a multiple of 4,
-// ng RR >= KC), and the multiplier that turns lane index t < NT into its
-// row group t / ncl = umulhi(2 t, mul). Read from constant memory with a
-// wave-uniform index, so ng and RR arrive in scalar registers.
-struct DotSplit {
-  unsigned ng_rr;  // ng | RR << 8
-  unsigned mul;    // ceil(2^31 / ncl)
-};
-template <int KC, int Q>
-struct DotSplitTab {
-  DotSplit e[HR<KC, Q>::L + 1];
-};
-template <int KC, int Q>
-constexpr DotSplitTab<KC, Q> make_dot_split() {
-  DotSplitTab<KC, Q> tab{};
-  for (int ncl = 1; ncl <= HR<KC, Q>::L; ++ncl) {
-    const int ng = HR<KC, Q>::NT / ncl < 16 ? HR<KC, Q>::NT / ncl : 16;
-    const int rr = ((KC + ng - 1) / ng + 3) & ~3;
-    tab.e[ncl].ng_rr = (unsigned)ng | ((unsigned)rr << 8);
-    tab.e[ncl].mul = (unsigned)(((1ull << 31) + ncl - 1) / ncl);
-  }
-  return tab;
-}
-// every lane index maps to its exact quotient
-template <int KC, int Q>
-constexpr bool dot_split_exact() {
-  constexpr DotSplitTab<KC, Q> tab = make_dot_split<KC, Q>();
-  for (int ncl = 1; ncl <= HR<KC, Q>::L; ++ncl)
-    for (unsigned t = 0; t < (unsigned)HR<KC, Q>::NT; ++t)
-      if ((unsigned)(((unsigned long long)(2 * t) * tab.e[ncl].mul) >> 32) != t / (unsigned)ncl)
-        return false;
-  return true;
-}
-static_assert(dot_split_exact<256, 2>() && dot_split_exact<128, 2>(), "lane split quotient");
-__constant__ DotSplitTab<256, 2> kDotSplit256 = make_dot_split<256, 2>();
-__constant__ DotSplitTab<128, 2> kDotSplit128 = make_dot_split<128, 2>();
-template <int KC, int Q>
-__device__ __forceinline__ const DotSplit &dot_split(int ncl) {
-  if constexpr (KC == 256)
-    return kDotSplit256.e[ncl];
-  else
-    return kDotSplit128.e[ncl];
-}
-
-// a0..a3 += v[u] * w[r + u] over N rows (N a multiple of 4), w from the
-// broadcast copy in LDS (FP32 lw, or its FP64 copy lwT), accumulators by
-// row mod 4
-template <int N, int KC, int Q, typename T>
-__device__ __forceinline__ void hr_dot_rows(const float (&v)[N], const HRShared<KC, Q, T> &sh,
-                                            int r, T &a0, T &a1, T &a2, T &a3) {
-  if constexpr (sizeof(T) == 8) {
-    const double2 *lw2 = reinterpret_cast<const double2 *>(sh.lwT + r);
-#pragma unroll
-    for (int u = 0; u < N / 4; ++u) {
-      const double2 l0 = lw2[2 * u], l1 = lw2[2 * u + 1];
-      a0 = fmaT((T)v[4 * u + 0], (T)l0.x, a0);
-      a1 = fmaT((T)v[4 * u + 1], (T)l0.y, a1);
-      a2 = fmaT((T)v[4 * u + 2], (T)l1.x, a2);
-      a3 = fmaT((T)v[4 * u + 3], (T)l1.y, a3);
-    }
-  } else {
-    const float4 *lw4 = reinterpret_cast<const float4 *>(sh.lw + r);
-#pragma unroll
-    for (int u = 0; u < N / 4; ++u) {
-      const float4 l = lw4[u];
-      a0 = fmaT((T)v[4 * u + 0], (T)l.x, a0);
-      a1 = fmaT((T)v[4 * u + 1], (T)l.y, a1);
-      a2 = fmaT((T)v[4 * u + 2], (T)l.z, a2);
-      a3 = fmaT((T)v[4 * u + 3], (T)l.w, a3);
-    }
-  }
-}
-
 // One classical Gram-Schmidt pass of w (row i's entry, identical on all row
 // parts) against basis columns 0..j; returns the new entry. With WANT_PRE,
 // ||w||^2 before the pass is returned in pre (its partials ride along with
@@ -1332,50 +1258,60 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     for (int u = 1; u < C::WPH; ++u) s += sh.rp[u];
     pre = s;
   }
-  // (1) LDS columns 0..ncl-1: lane t takes column t % ncl of row group
-  // t / ncl; ng = min(16, Q KC / ncl) groups of RR rows (RR a multiple of 4,
-  // ng RR >= KC), so nearly every lane works and the dependent chains stay
-  // short; rows >= k are zero
+  // (1) LDS columns 0..ncl-1, in groups of four adjacent columns: lane t
+  // (t < 8 ncg) takes column group cg = t / 8 over the rows r = 8 m + g,
+  // g = t % 8 (one 16-byte read of V per row feeds four FMAs; consecutive g
+  // are consecutive rows, so the eight lanes of a group hit distinct banks),
+  // and the eight lanes' partial sums meet by DPP; each group's first lane
+  // writes the four coefficients. Rows >= k and columns >= ncl are zero.
   const int ncl = min(j + 1, C::L);
-  const DotSplit sp = dot_split<KC, Q>(ncl);
-  const int ng = (int)(sp.ng_rr & 0xff), RR = (int)(sp.ng_rr >> 8);
   {
-    const int g = (int)__umulhi(2u * (unsigned)t, sp.mul), c = t - g * ncl;
-    if (g < ng) {
-      // RB rows per iteration (register budget: 16 FP32 and 4 FP64 at
-      // KC = 256, 8 at KC = 128 with its 168-VGPR cap), then 4 at a time: the
-      // column entries of a batch are all loaded before the FMAs consume
-      // them, so each batch pays the LDS latency once
-      constexpr int RB = KC == 256 ? (sizeof(T) == 8 ? 4 : 16) : 8;
-      const int r0 = g * RR;
-      const int rn = (min(RR, max(0, k - r0)) + 3) & ~3;
-      // ru = the rows of the last group, the fewest: a wave-uniform bound,
-      // so the main loop runs on a scalar counter; the tail is per lane
-      const int ru = __builtin_amdgcn_readfirstlane((min(RR, max(0, k - (ng - 1) * RR)) + 3) & ~3);
+    const int ncg = (ncl + 3) >> 2;
+    const int cg = t >> 3, g = t & 7;
+    if (cg < ncg) {
+      // batches of four rows; mm = rows / 8 rounded up to a multiple of 4
+      // (rows beyond k hold zeros, rows < KC exist)
+      const int mm = __builtin_amdgcn_readfirstlane((((k + 7) >> 3) + 3) & ~3);
+      const float *col = VL + g * C::S + 4 * cg;
       T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-      int r = 0;
-      for (; r + RB <= ru; r += RB) {
-        const float *col = VL + (r0 + r) * C::S + c;
-        float v[RB];
+      for (int m = 0; m < mm; m += 4) {
+        float4 v[4];
+        T l[4];
 #pragma unroll
-        for (int u = 0; u < RB; ++u) v[u] = col[u * C::S];
-        hr_dot_rows<RB, KC, Q, T>(v, sh, r0 + r, a0, a1, a2, a3);
-      }
-      for (; r + RB <= rn; r += RB) {
-        const float *col = VL + (r0 + r) * C::S + c;
-        float v[RB];
+        for (int u = 0; u < 4; ++u) {
+          v[u] = *reinterpret_cast<const float4 *>(col + (8 * (m + u)) * C::S);
+          if constexpr (sizeof(T) == 8)
+            l[u] = sh.lwT[8 * (m + u) + g];
+          else
+            l[u] = sh.lw[8 * (m + u) + g];
+        }
 #pragma unroll
-        for (int u = 0; u < RB; ++u) v[u] = col[u * C::S];
-        hr_dot_rows<RB, KC, Q, T>(v, sh, r0 + r, a0, a1, a2, a3);
+        for (int u = 0; u < 4; ++u) {
+          a0 = fmaT((T)v[u].x, l[u], a0);
+          a1 = fmaT((T)v[u].y, l[u], a1);
+          a2 = fmaT((T)v[u].z, l[u], a2);
+          a3 = fmaT((T)v[u].w, l[u], a3);
+        }
       }
-      for (; r < rn; r += 4) {
-        const float *col = VL + (r0 + r) * C::S + c;
-        float v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = col[u * C::S];
-        hr_dot_rows<4, KC, Q, T>(v, sh, r0 + r, a0, a1, a2, a3);
+      a0 += dpp<DPP_QUAD_1032>(a0);
+      a1 += dpp<DPP_QUAD_1032>(a1);
+      a2 += dpp<DPP_QUAD_1032>(a2);
+      a3 += dpp<DPP_QUAD_1032>(a3);
+      a0 += dpp<DPP_QUAD_2301>(a0);
+      a1 += dpp<DPP_QUAD_2301>(a1);
+      a2 += dpp<DPP_QUAD_2301>(a2);
+      a3 += dpp<DPP_QUAD_2301>(a3);
+      a0 += dpp<DPP_ROW_HMIRROR>(a0);
+      a1 += dpp<DPP_ROW_HMIRROR>(a1);
+      a2 += dpp<DPP_ROW_HMIRROR>(a2);
+      a3 += dpp<DPP_ROW_HMIRROR>(a3);
+      if (g == 0) {
+        T *cq = sh.cp + 4 * cg;
+        cq[0] = a0;
+        cq[1] = a1;
+        cq[2] = a2;
+        cq[3] = a3;
       }
-      sh.cp[g * ncl + c] = (a0 + a1) + (a2 + a3);
     }
   }
   // (2) register columns L + Q m + h (m < nm): products reduce-scattered per
@@ -1400,13 +1336,13 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     }
   }
   __syncthreads();
-  // (3) the coefficients, lane t < KC for column t (zero beyond j); the
-  // partials are summed in group (wavefront) order
+  // (3) the coefficients, lane t < KC for column t (zero beyond j): LDS
+  // columns as the column groups wrote them, register columns summed over
+  // the part's wavefronts in wavefront order
   if (h == 0) {
     T hc = 0;
     if (i < ncl) {
       hc = sh.cp[i];
-      for (int g = 1; g < ng; ++g) hc += sh.cp[g * ncl + i];
     } else if constexpr (C::R > 0) {
       if (i >= C::L && i <= j) {
         // column i is the m-th register column of part hh, in slot
