@@ -1371,26 +1371,31 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     // 8 columns per iteration, then at most one group of 4; reads stay in
     // [c0, c0 + cn rounded up to 4), inside this part's share of the row and
     // below L (columns past j hold zeros and zero coefficients)
+    // the coefficients are read 16 bytes at a time too (c0 is a multiple of
+    // 4; the part is uniform per wavefront, so these reads are broadcasts)
     const int cn4 = (cn + 3) & ~3;
     int c = 0;
     for (; c + 8 <= cn4; c += 8) {
       const float4 ra = *reinterpret_cast<const float4 *>(row + c);
       const float4 rb = *reinterpret_cast<const float4 *>(row + c + 4);
-      u0 = __builtin_fmaf(ra.x, hl[c + 0], u0);
-      u1 = __builtin_fmaf(ra.y, hl[c + 1], u1);
-      u2 = __builtin_fmaf(ra.z, hl[c + 2], u2);
-      u3 = __builtin_fmaf(ra.w, hl[c + 3], u3);
-      u0 = __builtin_fmaf(rb.x, hl[c + 4], u0);
-      u1 = __builtin_fmaf(rb.y, hl[c + 5], u1);
-      u2 = __builtin_fmaf(rb.z, hl[c + 6], u2);
-      u3 = __builtin_fmaf(rb.w, hl[c + 7], u3);
+      const float4 ha = *reinterpret_cast<const float4 *>(hl + c);
+      const float4 hb = *reinterpret_cast<const float4 *>(hl + c + 4);
+      u0 = __builtin_fmaf(ra.x, ha.x, u0);
+      u1 = __builtin_fmaf(ra.y, ha.y, u1);
+      u2 = __builtin_fmaf(ra.z, ha.z, u2);
+      u3 = __builtin_fmaf(ra.w, ha.w, u3);
+      u0 = __builtin_fmaf(rb.x, hb.x, u0);
+      u1 = __builtin_fmaf(rb.y, hb.y, u1);
+      u2 = __builtin_fmaf(rb.z, hb.z, u2);
+      u3 = __builtin_fmaf(rb.w, hb.w, u3);
     }
     if (c < cn4) {
       const float4 ra = *reinterpret_cast<const float4 *>(row + c);
-      u0 = __builtin_fmaf(ra.x, hl[c + 0], u0);
-      u1 = __builtin_fmaf(ra.y, hl[c + 1], u1);
-      u2 = __builtin_fmaf(ra.z, hl[c + 2], u2);
-      u3 = __builtin_fmaf(ra.w, hl[c + 3], u3);
+      const float4 ha = *reinterpret_cast<const float4 *>(hl + c);
+      u0 = __builtin_fmaf(ra.x, ha.x, u0);
+      u1 = __builtin_fmaf(ra.y, ha.y, u1);
+      u2 = __builtin_fmaf(ra.z, ha.z, u2);
+      u3 = __builtin_fmaf(ra.w, ha.w, u3);
     }
   }
   if constexpr (C::R > 0) {
