@@ -438,6 +438,12 @@ class Server:
         _, _, out = self._pending.pop(ticket)
         return out, lat.value, q.value
 
+    def poll(self, ticket):
+        """True once the request is complete (non-blocking)."""
+        done = ctypes.c_int()
+        check(load().pl_server_poll(self._h, int(ticket), ctypes.byref(done)), "pl_server_poll")
+        return bool(done.value)
+
     def stats(self, reset=False):
         from ._lib import ServerStats
         st = ServerStats()
