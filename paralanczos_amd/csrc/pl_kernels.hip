@@ -1259,31 +1259,42 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     pre = s;
   }
   // (1) LDS columns 0..ncl-1, in groups of four adjacent columns: lane t
-  // (t < 8 ncg) takes column group cg = t / 8 over the rows r = 8 m + g,
-  // g = t % 8 (one 16-byte read of V per row feeds four FMAs; consecutive g
-  // are consecutive rows, so the eight lanes of a group hit distinct banks),
-  // and the eight lanes' partial sums meet by DPP; each group's first lane
-  // writes the four coefficients. Rows >= k and columns >= ncl are zero.
+  // (t < 8 ncg) takes column group cg = t / 8 over the row pairs
+  // 16 b + 2 g, 16 b + 2 g + 1 (g = t % 8): one 16-byte read of V per row
+  // feeds four FMAs, and one read of w serves both rows of a pair. With the
+  // row stride S (S / 4 odd), the even rows of the eight lanes fall in
+  // distinct even bank quads and the odd rows in odd ones, so the two column
+  // groups of a quarter wavefront read disjoint banks. The eight lanes'
+  // partial sums meet by DPP, and each group's first lane writes the four
+  // coefficients. Rows >= k and columns >= ncl are zero.
   const int ncl = min(j + 1, C::L);
   {
+    static_assert((C::S / 4) % 2 == 1, "bank layout of the dot reads");
     const int ncg = (ncl + 3) >> 2;
     const int cg = t >> 3, g = t & 7;
     if (cg < ncg) {
-      // batches of four rows; mm = rows / 8 rounded up to a multiple of 4
-      // (rows beyond k hold zeros, rows < KC exist)
-      const int mm = __builtin_amdgcn_readfirstlane((((k + 7) >> 3) + 3) & ~3);
-      const float *col = VL + g * C::S + 4 * cg;
+      // batches of two 16-row blocks (four rows per lane); nb = 16-row
+      // blocks rounded up to even (rows beyond k hold zeros, rows < KC exist)
+      const int nb = __builtin_amdgcn_readfirstlane((((k + 15) >> 4) + 1) & ~1);
+      const float *col = VL + 2 * g * C::S + 4 * cg;
       T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-      for (int m = 0; m < mm; m += 4) {
+      for (int b = 0; b < nb; b += 2) {
         float4 v[4];
         T l[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          v[u] = *reinterpret_cast<const float4 *>(col + (8 * (m + u)) * C::S);
-          if constexpr (sizeof(T) == 8)
-            l[u] = sh.lwT[8 * (m + u) + g];
-          else
-            l[u] = sh.lw[8 * (m + u) + g];
+        for (int u = 0; u < 2; ++u) {
+          const int r = 16 * (b + u);
+          v[2 * u] = *reinterpret_cast<const float4 *>(col + r * C::S);
+          v[2 * u + 1] = *reinterpret_cast<const float4 *>(col + (r + 1) * C::S);
+          if constexpr (sizeof(T) == 8) {
+            const double2 x = *reinterpret_cast<const double2 *>(sh.lwT + r + 2 * g);
+            l[2 * u] = x.x;
+            l[2 * u + 1] = x.y;
+          } else {
+            const float2 x = *reinterpret_cast<const float2 *>(sh.lw + r + 2 * g);
+            l[2 * u] = x.x;
+            l[2 * u + 1] = x.y;
+          }
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
