@@ -553,6 +553,20 @@ def main():
     torch.cuda.synchronize()
     dom_ms = e0.elapsed_time(e1) / 3
     del ws
+    # the second kernel of the mix: stage 1 over as many k = 128 subsets as
+    # the step holds (the 256-lane kernel, FP64 coefficients)
+    n128 = int(sum(int((q["ks"] == 128).sum().item()) for q in reqs))
+    idx128 = solver.gen_subsets(SEED_SAMPLE + 1, n128, 128, N)
+    ws = torch.empty(solver.workspace_size(n128, 128), dtype=torch.uint8, device=dev)
+    o128 = solver.solve_stage(G, idx128, 1, ws, k=128)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(3):
+        solver.solve_stage(G, idx128, 1, ws, k=128, out=o128)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    k128_ms = e0.elapsed_time(e1) / 3
+    del ws, idx128, o128
 
     srv = solver.Server(G, max_subsets=16384, max_indices=16384 * 256, max_requests=1024,
                         max_k=256, depth=3)
@@ -828,6 +842,12 @@ def main():
                      "traffic_split": traffic_split,
                      "kernel": "solve_rowpart_kernel<256, 2, TRIDIAG, double>",
                      "launch_ms": dom_ms, "subsets_per_launch": n256},
+        "roofline_k128": {"bound": "valu", "kernel": "solve_rowpart_kernel<128, 2, TRIDIAG, double>",
+                          "launch_ms": k128_ms, "subsets_per_launch": n128,
+                          "achieved": n128 * flops_per_solve(128) / (k128_ms / 1e3) / 1e12,
+                          "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": n128 * flops_per_solve(128) / (k128_ms / 1e3) / 1e12
+                          / FP32_PEAK_TFLOPS},
         "roofline_hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS},
         "roofline_lds": lds,
