@@ -352,6 +352,68 @@ def gram_leg(solver, torch, phi, G):
             "library_gemm_ms": lib_ms, "vs_library": lib_ms / ms, "max_abs_diff_vs_library": diff}
 
 
+def host_buffers_leg(solver, torch, G, idx_all, off_all, tot, reps=3):
+    """Rates with the inputs and outputs in host memory (PCIe included).
+    main: the step's CSR batch with indices, offsets and the four outputs in
+    pinned host memory; each call uploads the indices and offsets, solves the
+    tuned form on the resident G and downloads the outputs, on torch's stream.
+    legacy_c1 / legacy_c2: pl_multi_subsvd (the reference's host entry point)
+    on the uniform configurations, FP64 G and int32 indices in host memory,
+    all eigenvalues back; each call converts G to FP32, allocates, uploads,
+    solves and downloads."""
+    from paralanczos_amd import legacy
+
+    res = {}
+    idx_h = torch.empty(idx_all.shape, dtype=idx_all.dtype, pin_memory=True)
+    off_h = torch.empty(off_all.shape, dtype=off_all.dtype, pin_memory=True)
+    idx_h.copy_(idx_all)
+    off_h.copy_(off_all)
+    idx_d, off_d = torch.empty_like(idx_all), torch.empty_like(off_all)
+    o = solver.solve(G, idx_all, k=256, offsets=off_all)
+    oh = [torch.empty(t.shape, dtype=t.dtype, pin_memory=True) for t in o]
+
+    def call():
+        idx_d.copy_(idx_h, non_blocking=True)
+        off_d.copy_(off_h, non_blocking=True)
+        solver.solve(G, idx_d, k=256, offsets=off_d, out=o)
+        for a, b in zip(oh, o):
+            a.copy_(b, non_blocking=True)
+
+    call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        solver.solve(G, idx_all, k=256, offsets=off_all, out=o)
+    torch.cuda.synchronize()
+    ms_dev = (time.perf_counter() - t0) * 1e3 / reps
+    moved = idx_h.numel() * 4 + off_h.numel() * 8 + sum(t.numel() * 4 for t in oh)
+    res["main"] = {"value": tot / (ms / 1e3), "ms": ms, "device_resident_ms": ms_dev,
+                   "bytes_moved": moved, "subsets": tot,
+                   "workload": "the step's CSR batch, pinned host indices/offsets/outputs"}
+    del idx_h, off_h, idx_d, off_d, o, oh
+    for name, seed, m, n, k, nsub in (("legacy_c1", 0xC1, 64, 256, 8, 1024),
+                                      ("legacy_c2", 0xC2, 512, 4096, 32, 1 << 20)):
+        g = solver.gram(solver.gen_phi("gauss", seed, m, n)).double().cpu().numpy()
+        ix = solver.gen_subsets(seed + 1, nsub, k, n).cpu().numpy()
+        eigs, _, err = legacy.gpu_multi_subsvd(g, ix, k)  # warm
+        n_rep = 20 if nsub < 4096 else 3
+        t0 = time.perf_counter()
+        for _ in range(n_rep):
+            legacy.gpu_multi_subsvd(g, ix, k)
+        ms = (time.perf_counter() - t0) * 1e3 / n_rep
+        res[name] = {"value": nsub / (ms / 1e3), "ms": ms, "subsets": nsub, "k": k,
+                     "err_nonzero": int(np.count_nonzero(err)),
+                     "workload": f"pl_multi_subsvd, FP64 G {n}^2 and {nsub} subsets of k={k} "
+                                 f"in host memory, all eigenvalues returned"}
+    torch.cuda.empty_cache()
+    return res
+
+
 def kernel_times(solver, torch, dev):
     """Stage 1 of the k = 128 and k = 256 kernels on the serving G: 6200
     subsets each (one served wave's worth), FP64 and FP32 Gram-Schmidt
@@ -719,6 +781,7 @@ def main():
         legs.update(small_configs_leg(solver, torch))
         legs["batching"] = batching_leg(solver, torch)
         legs["precision"] = precision_leg(solver, torch, dev)
+        legs["host_buffers"] = host_buffers_leg(solver, torch, G, idx_all, off_all, tot)
 
     cpu = None
     if not args.no_cpu:
