@@ -1053,7 +1053,7 @@ struct HRShared {
   };
   union {
     float lvn[KC];      // next Lanczos vector, unnormalised: the matvec input
-    T cp[(C::L + 3) / 4 * 4];  // LDS-column coefficients, written by the column groups
+    T cp[32 * C::NW];   // LDS-column coefficient partials: four per dot unit (8 NW units)
     double redd[16];    // bounds reduction, after the recurrence
   };
   double ra[C::NW];     // alpha partials (FP64) of all wavefronts
@@ -1258,50 +1258,91 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     for (int u = 1; u < C::WPH; ++u) s += sh.rp[u];
     pre = s;
   }
-  // (1) LDS columns 0..ncl-1, in groups of four adjacent columns: lane t
-  // (t < 8 ncg) takes column group cg = t / 8 over the row pairs
-  // 16 b + 2 g, 16 b + 2 g + 1 (g = t % 8): one 16-byte read of V per row
-  // feeds four FMAs, and one read of w serves both rows of a pair. With the
-  // row stride S (S / 4 odd), the even rows of the eight lanes fall in
-  // distinct even bank quads and the odd rows in odd ones, so the two column
-  // groups of a quarter wavefront read disjoint banks. The eight lanes'
-  // partial sums meet by DPP, and each group's first lane writes the four
-  // coefficients. Rows >= k and columns >= ncl are zero.
+  // (1) LDS columns 0..ncl-1, in groups of four adjacent columns. A unit
+  // is a group over a part of its rows, taken by eight lanes of one slot
+  // (slot = t / 8 mod 8) of a wavefront; wavefront w takes row part
+  // w mod P of the eight consecutive groups 8 (w / P) + slot. At KC = 256,
+  // P is the largest power of two <= 8 with ncg P <= 64 (a part holds at
+  // least two 16-row blocks), so a step with few columns spreads its dots
+  // over every SIMD and both wavefronts of each; the part is uniform per
+  // wavefront and adjacent slots take adjacent groups. Past 32 groups
+  // wavefronts 0..3 take groups 0..31 whole and wavefronts 4..7 the rows of
+  // the remaining groups in four parts, beside their share of the register
+  // columns. At KC = 128 (three subsets per CU) P = 1. A unit's eight lanes
+  // take its four columns over the row pairs 16 b + 2 g, 16 b + 2 g + 1
+  // (g = t % 8) of its blocks: one 16-byte read of V per row feeds four
+  // FMAs, and one read of w serves both rows of a pair. With the row stride
+  // S (S / 4 odd), the even rows of the eight lanes fall in distinct even
+  // bank quads and the odd rows in odd ones, so the two units of a quarter
+  // wavefront read disjoint banks. The eight lanes' partial sums meet by
+  // DPP, and each unit's first lane writes its four partials, part-major
+  // (part p of column c at p (256 / P) + c; the tail parts at 128 + 32 p +
+  // c - 128), so the coefficient sums read each part with consecutive lanes
+  // on consecutive words and add the parts by a fixed tree. Rows >= k and
+  // columns >= ncl are zero.
   const int ncl = min(j + 1, C::L);
+  const int ncg = (ncl + 3) >> 2;
+  constexpr int SL = 8 * C::NW, PMAX = KC == 256 ? 8 : 1;
+  const bool split_tail = C::NW == 8 && ncg > 32;  // wave-uniform
+  int lp = 0;  // log2 P, wave-uniform
+  while ((2 << lp) <= PMAX && ncg * (2 << lp) <= SL) ++lp;
+  lp = __builtin_amdgcn_readfirstlane(lp);
   {
     static_assert((C::S / 4) % 2 == 1, "bank layout of the dot reads");
-    const int ncg = (ncl + 3) >> 2;
-    const int cg = t >> 3, g = t & 7;
+    static_assert(PMAX <= C::NW, "a wavefront takes one row part");
+    const int wv0 = t >> 6, sl = (t >> 3) & 7, g = t & 7;
+    // 16-row blocks rounded up to even (rows beyond k hold zeros, rows < KC
+    // exist); a part's blocks [b0, b1), nbp = blocks per part, even
+    const int nbe = (((k + 15) >> 4) + 1) & ~1;
+    int cg, wo, b0, b1;  // group, partial offset of its column 4 cg, blocks
+    if (!split_tail) {
+      const int pp = wv0 & ((1 << lp) - 1);
+      const int nbp = (((nbe + (1 << lp) - 1) >> lp) + 1) & ~1;
+      cg = 8 * (wv0 >> lp) + sl;
+      wo = pp * ((4 * SL) >> lp) + 4 * cg;
+      b0 = pp * nbp;
+      b1 = min(b0 + nbp, nbe);
+    } else if (wv0 < 4) {
+      cg = 8 * wv0 + sl;
+      wo = 4 * cg;
+      b0 = 0;
+      b1 = nbe;
+    } else {
+      const int pp = wv0 - 4, nbp = (((nbe + 3) >> 2) + 1) & ~1;
+      cg = 32 + sl;
+      wo = 128 + 32 * pp + 4 * sl;
+      b0 = pp * nbp;
+      b1 = min(b0 + nbp, nbe);
+    }
+    b0 = __builtin_amdgcn_readfirstlane(b0);
+    b1 = __builtin_amdgcn_readfirstlane(b1);
     if (cg < ncg) {
-      // batches of two 16-row blocks (four rows per lane); nb = 16-row
-      // blocks rounded up to even (rows beyond k hold zeros, rows < KC exist)
-      const int nb = __builtin_amdgcn_readfirstlane((((k + 15) >> 4) + 1) & ~1);
       const float *col = VL + 2 * g * C::S + 4 * cg;
       T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-      for (int b = 0; b < nb; b += 2) {
+      for (int b = b0; b < b1; b += 2) {
         float4 v[4];
         T l[4];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int r = 16 * (b + u);
-          v[2 * u] = *reinterpret_cast<const float4 *>(col + r * C::S);
-          v[2 * u + 1] = *reinterpret_cast<const float4 *>(col + (r + 1) * C::S);
+        for (int u2 = 0; u2 < 2; ++u2) {
+          const int r = 16 * (b + u2);
+          v[2 * u2] = *reinterpret_cast<const float4 *>(col + r * C::S);
+          v[2 * u2 + 1] = *reinterpret_cast<const float4 *>(col + (r + 1) * C::S);
           if constexpr (sizeof(T) == 8) {
             const double2 x = *reinterpret_cast<const double2 *>(sh.lwT + r + 2 * g);
-            l[2 * u] = x.x;
-            l[2 * u + 1] = x.y;
+            l[2 * u2] = x.x;
+            l[2 * u2 + 1] = x.y;
           } else {
             const float2 x = *reinterpret_cast<const float2 *>(sh.lw + r + 2 * g);
-            l[2 * u] = x.x;
-            l[2 * u + 1] = x.y;
+            l[2 * u2] = x.x;
+            l[2 * u2 + 1] = x.y;
           }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          a0 = fmaT((T)v[u].x, l[u], a0);
-          a1 = fmaT((T)v[u].y, l[u], a1);
-          a2 = fmaT((T)v[u].z, l[u], a2);
-          a3 = fmaT((T)v[u].w, l[u], a3);
+        for (int u2 = 0; u2 < 4; ++u2) {
+          a0 = fmaT((T)v[u2].x, l[u2], a0);
+          a1 = fmaT((T)v[u2].y, l[u2], a1);
+          a2 = fmaT((T)v[u2].z, l[u2], a2);
+          a3 = fmaT((T)v[u2].w, l[u2], a3);
         }
       }
       a0 += dpp<DPP_QUAD_1032>(a0);
@@ -1317,7 +1358,7 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
       a2 += dpp<DPP_ROW_HMIRROR>(a2);
       a3 += dpp<DPP_ROW_HMIRROR>(a3);
       if (g == 0) {
-        T *cq = sh.cp + 4 * cg;
+        T *cq = sh.cp + wo;
         cq[0] = a0;
         cq[1] = a1;
         cq[2] = a2;
@@ -1348,12 +1389,29 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   }
   __syncthreads();
   // (3) the coefficients, lane t < KC for column t (zero beyond j): LDS
-  // columns as the column groups wrote them, register columns summed over
-  // the part's wavefronts in wavefront order
+  // columns as the sum of their group's row parts in part order, register
+  // columns summed over the part's wavefronts in wavefront order
   if (h == 0) {
     T hc = 0;
     if (i < ncl) {
-      hc = sh.cp[i];
+      // the column's parts, by a pairwise tree: P of them (a wave-uniform
+      // count) at i + p 4 SL / P, or the four tail parts at i + 32 p
+      if (split_tail && i >= 128) {
+        const T *cq = sh.cp + i;
+        hc = (cq[0] + cq[32]) + (cq[64] + cq[96]);
+      } else if (lp == 0) {
+        hc = sh.cp[i];
+      } else if (lp == 1) {
+        hc = sh.cp[i] + sh.cp[i + 2 * SL];
+      } else if (lp == 2) {
+        const T *cq = sh.cp + i;
+        hc = (cq[0] + cq[SL]) + (cq[2 * SL] + cq[3 * SL]);
+      } else {
+        const T *cq = sh.cp + i;
+        const int st = SL / 2;
+        hc = ((cq[0] + cq[st]) + (cq[2 * st] + cq[3 * st])) +
+             ((cq[4 * st] + cq[5 * st]) + (cq[6 * st] + cq[7 * st]));
+      }
     } else if constexpr (C::R > 0) {
       if (i >= C::L && i <= j) {
         // column i is the m-th register column of part hh, in slot
