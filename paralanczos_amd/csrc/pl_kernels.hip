@@ -1469,18 +1469,29 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   }
   if constexpr (C::R > 0) {
     // slots 0..nm_h-1 of this part, coefficients read 4 at a time in slot
-    // order (a broadcast: the part is uniform per wavefront)
+    // order (a broadcast: the part is uniform per wavefront), the four reads
+    // of a block of 16 slots issued together (slots past nm_h hold zeros)
+    constexpr int NG = (C::R + 3) / 4;  // groups of 4 slots
     const int hu = __builtin_amdgcn_readfirstlane(h);
     const int nmh = j >= C::L + hu ? (j - C::L - hu) / Q + 1 : 0;
     const float4 *lr = reinterpret_cast<const float4 *>(sh.lhr[hu]);
 #pragma unroll
-    for (int m0 = 0; m0 < C::R; m0 += 4) {
-      if (m0 < nmh) {  // uniform; slots past nm_h hold zeros
-        const float4 cf = lr[m0 / 4];
-        u0 = __builtin_fmaf(vr[m0], cf.x, u0);
-        if (m0 + 1 < C::R) u1 = __builtin_fmaf(vr[m0 + 1], cf.y, u1);
-        if (m0 + 2 < C::R) u2 = __builtin_fmaf(vr[m0 + 2], cf.z, u2);
-        if (m0 + 3 < C::R) u3 = __builtin_fmaf(vr[m0 + 3], cf.w, u3);
+    for (int g0 = 0; g0 < NG; g0 += 4) {
+      if (4 * g0 < nmh) {  // uniform
+        float4 cf[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (g0 + q < NG) cf[q] = lr[g0 + q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int m0 = 4 * (g0 + q);
+          if (g0 + q < NG && m0 < nmh) {
+            u0 = __builtin_fmaf(vr[m0], cf[q].x, u0);
+            if (m0 + 1 < C::R) u1 = __builtin_fmaf(vr[m0 + 1], cf[q].y, u1);
+            if (m0 + 2 < C::R) u2 = __builtin_fmaf(vr[m0 + 2], cf[q].z, u2);
+            if (m0 + 3 < C::R) u3 = __builtin_fmaf(vr[m0 + 3], cf[q].w, u3);
+          }
+        }
       }
     }
   }
