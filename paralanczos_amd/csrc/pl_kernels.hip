@@ -1437,13 +1437,30 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     const int cn = __builtin_amdgcn_readfirstlane(max(0, min(cs, ncl - c0)));  // per part
     const float *row = VL + i * C::S + c0;
     const float *hl = sh.lh + c0;
-    // 8 columns per iteration, then at most one group of 4; reads stay in
+    // 16 (FP32) or 8 columns per iteration, then at most one group of 4; reads stay in
     // [c0, c0 + cn rounded up to 4), inside this part's share of the row and
     // below L (columns past j hold zeros and zero coefficients)
     // the coefficients are read 16 bytes at a time too (c0 is a multiple of
     // 4; the part is uniform per wavefront, so these reads are broadcasts)
     const int cn4 = (cn + 3) & ~3;
     int c = 0;
+    // 16 columns per round trip (FP32 coefficients: the FP64 form has no
+    // registers to spare for them)
+    for (; sizeof(T) == 4 && c + 16 <= cn4; c += 16) {
+      float4 ra[4], ha[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        ra[q] = *reinterpret_cast<const float4 *>(row + c + 4 * q);
+        ha[q] = *reinterpret_cast<const float4 *>(hl + c + 4 * q);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        u0 = __builtin_fmaf(ra[q].x, ha[q].x, u0);
+        u1 = __builtin_fmaf(ra[q].y, ha[q].y, u1);
+        u2 = __builtin_fmaf(ra[q].z, ha[q].z, u2);
+        u3 = __builtin_fmaf(ra[q].w, ha[q].w, u3);
+      }
+    }
     for (; c + 8 <= cn4; c += 8) {
       const float4 ra = *reinterpret_cast<const float4 *>(row + c);
       const float4 rb = *reinterpret_cast<const float4 *>(row + c + 4);
