@@ -1170,14 +1170,22 @@ __device__ __forceinline__ double hr_minmax(double x, int slot, HRShared<KC, Q, 
 // broadcast; four accumulators by index mod 4
 template <int N>
 __device__ __forceinline__ float hr_rowdot(const float (&g)[N], const float *L) {
+  static_assert(N % 16 == 0, "blocks of 16 entries");
   float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+  // four 16-byte reads (16 entries) issued together per block
 #pragma unroll
-  for (int q = 0; q < N / 4; ++q) {
-    const float4 t = reinterpret_cast<const float4 *>(L)[q];
-    a0 = __builtin_fmaf(g[4 * q + 0], t.x, a0);
-    a1 = __builtin_fmaf(g[4 * q + 1], t.y, a1);
-    a2 = __builtin_fmaf(g[4 * q + 2], t.z, a2);
-    a3 = __builtin_fmaf(g[4 * q + 3], t.w, a3);
+  for (int b = 0; b < N / 16; ++b) {
+    float4 t[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) t[u] = reinterpret_cast<const float4 *>(L)[4 * b + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = 4 * b + u;
+      a0 = __builtin_fmaf(g[4 * q + 0], t[u].x, a0);
+      a1 = __builtin_fmaf(g[4 * q + 1], t[u].y, a1);
+      a2 = __builtin_fmaf(g[4 * q + 2], t[u].z, a2);
+      a3 = __builtin_fmaf(g[4 * q + 3], t[u].w, a3);
+    }
   }
   return (a0 + a1) + (a2 + a3);
 }
