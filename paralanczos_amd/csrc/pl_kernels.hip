@@ -39,6 +39,7 @@
 #include <map>
 #include <mutex>
 #include <math.h>
+#include <stddef.h>
 
 #include "pl_common.hpp"
 #include "pl_internal.hpp"
@@ -1011,14 +1012,26 @@ struct HR {
   static constexpr int HW = KC / Q;             // G_S columns per lane
   static constexpr int WPH = KC / 64;           // wavefronts per row part
   static constexpr int NW = Q * WPH;            // wavefronts
+  static constexpr int Q_ = Q;
   static constexpr int LOGK = KC == 256 ? 8 : 7;
   static constexpr int L = KC == 256 ? 148 : 92;  // basis columns in LDS
   static constexpr int S = KC == 256 ? 148 : 92;  // LDS row stride: 16-byte rows, stride/4 odd
-  static constexpr int R = (KC - L) / Q;        // register basis columns per lane
-  static constexpr int RA = R > 0 ? R : 1;      // their array extent
+  static constexpr int R = (KC - L) / Q;        // register basis columns per part
+  // KC = 256: the register columns of a part are held in blocks of eight
+  // rows. Lane e of the block of rows 8b..8b+7 holds the part's columns m
+  // with m mod 8 = e (slot s = m / 8), entry p of a slot being row
+  // 8b + (p xor e). A column's dot then sums eight rows inside the lane, so
+  // the wavefront reduce-scatter moves one value per slot instead of one per
+  // column, and the xor order lets the update's per-row partials meet by
+  // three DPP adds with no selects. KC = 128 keeps one row per lane.
+  static constexpr bool OCT = KC == 256;
+  static constexpr int RB = OCT ? (R + 7) / 8 : R;  // register slots per lane
+  static constexpr int RA = OCT ? 8 * RB : (R > 0 ? R : 1);  // their array extent
   static_assert(S % 4 == 0 && (S / 4) % 2 == 1 && L % 4 == 0 && L <= S, "row layout");
+  static_assert(!OCT || (Q == 2 && NW == 8 && RB <= 8), "8-row blocks: slots fit one reduce-scatter");
 };
 constexpr int DPP_ROW_ROR8 = 0x128;  // lane i <- lane (i + 8) mod 16 within a row
+constexpr int DPP_QUAD_3210 = 0x1B;  // quad_perm [3,2,1,0]  (partner lane ^ 3)
 
 // reduce-scatter chunk: 16 FP32 or 8 FP64 values (register budget)
 template <typename T>
@@ -1037,25 +1050,38 @@ constexpr int k2_chunk() {
 template <int KC, int Q, typename T>
 struct HRShared {
   using C = HR<KC, Q>;
-  static constexpr int RP = C::R > 0 ? (C::R + k2_chunk<T>() - 1) / k2_chunk<T>() * k2_chunk<T>()
+  static constexpr int RP = C::OCT ? 8 * C::RB
+                          : C::R > 0 ? (C::R + k2_chunk<T>() - 1) / k2_chunk<T>() * k2_chunk<T>()
                                      : 1;  // register partial slots per (part, wavefront)
-  union {
+  static constexpr int RPN = C::Q_ * C::WPH * RP;
+  // LDS-column coefficient partials: four per dot unit (8 NW units); at
+  // KC = 256 also the tail groups' eight row parts (256 + TST p + c - 128)
+  static constexpr int TST = (C::L - 128 + 3) / 4 * 4;  // tail columns, rounded to groups
+  static constexpr int CPN = KC == 256 ? 256 + 8 * TST : 32 * C::NW;
+  // every buffer read 16 bytes at a time starts on a 16-byte boundary
+  union alignas(16) {
     float lw[KC];    // w broadcast
     float lh[KC];    // Gram-Schmidt coefficients (accumulated in T, applied in FP32)
   };
-  T lwT[sizeof(T) == 8 ? KC : 4];  // w in T (FP64 coefficients only): one conversion per entry
-  // the register columns' coefficients per part in slot order (16-byte reads)
-  alignas(16) float lhr[Q][C::R > 0 ? (C::R + 3) / 4 * 4 : 4];
-  union {
+  // the register columns' coefficients per part (16-byte reads): in slot
+  // order, or at KC = 256 by [lane of the block][slot]
+  alignas(16) float lhr[Q][C::OCT ? 64 : C::R > 0 ? (C::R + 3) / 4 * 4 : 4];
+  union alignas(16) {
     float pw[Q][KC];    // matvec partials of the row parts
     float part[Q][KC];  // Gram-Schmidt update partials of the row parts
-    T rpart[Q][C::WPH][RP];  // register-column partials per part and wavefront
+    // register-column partials per part and wavefront [h][wv][m] (KC = 128:
+    // written after the broadcast of w, when pw is dead)
+    T rpart_u[C::OCT ? 1 : RPN];
   };
-  union {
+  // KC = 256: the register-column partials are written before the barrier
+  // that publishes w, while other wavefronts may still read pw
+  alignas(16) T rpart_s[C::OCT ? RPN : 1];
+  union alignas(16) {
     float lvn[KC];      // next Lanczos vector, unnormalised: the matvec input
-    T cp[32 * C::NW];   // LDS-column coefficient partials: four per dot unit (8 NW units)
+    T cp[CPN];          // LDS-column coefficient partials
     double redd[16];    // bounds reduction, after the recurrence
   };
+  __device__ T *rpart(int h, int wv) { return (C::OCT ? rpart_s : rpart_u) + (h * C::WPH + wv) * RP; }
   double ra[C::NW];     // alpha partials (FP64) of all wavefronts
   double rb[C::WPH];    // beta^2 partials (FP64) of the wavefronts of part 0
   float rp[C::WPH];     // ||w||^2 partials before a Gram-Schmidt pass
@@ -1070,6 +1096,16 @@ struct HRLds {
   HRShared<KC, Q, T> sh;
   alignas(16) float VL[KC * HR<KC, Q>::S];
 };
+template <int KC, int Q, typename T>
+constexpr bool hr_aligned() {
+  using SH = HRShared<KC, Q, T>;
+  using LD = HRLds<KC, Q, T>;
+  return offsetof(SH, lvn) % 16 == 0 && offsetof(SH, lw) % 16 == 0 && offsetof(SH, lhr) % 16 == 0 &&
+         offsetof(SH, pw) % 16 == 0 && offsetof(LD, VL) % 16 == 0;
+}
+static_assert(hr_aligned<256, 2, float>() && hr_aligned<256, 2, double>() &&
+                  hr_aligned<128, 2, float>() && hr_aligned<128, 2, double>(),
+              "16-byte reads of the work buffers");
 template <int KC, int Q>
 constexpr size_t hr_lds_bytes() {
   return sizeof(HRLds<KC, Q, double>);
@@ -1150,6 +1186,50 @@ __device__ __forceinline__ T wave_rs(const T (&x)[N], int q) {
   return r;
 }
 
+// Reduce-scatter of 8 values over the eight 8-lane blocks of a wavefront
+// (lanes that agree in bits 0..2): lane q returns the sum over its eight
+// block partners of x[(q >> 3) & 7]. Partners by stage: ^32 and ^16
+// (permlane swaps: the lower half keeps the first operand), then ^8
+// (row_ror:8).
+template <typename T>
+__device__ __forceinline__ T blk_rs(const T (&x)[8], int q) {
+  T y[4], z[2];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) y[u] = swap_add<T>(x[u], x[u + 4], true);
+#pragma unroll
+  for (int u = 0; u < 2; ++u) z[u] = swap_add<T>(y[u], y[u + 2], false);
+  const bool b3 = (q >> 3) & 1;
+  const T send = b3 ? z[0] : z[1];
+  const T keep = b3 ? z[1] : z[0];
+  return keep + dpp<DPP_ROW_ROR8>(send);
+}
+
+// Sum over the eight lanes of a block of their x[p xor (lane mod 8)]: lane e
+// holds in x[p] its partial for row p xor e of the block and gets the total
+// of row e. Partners ^7 (row_half_mirror), ^2 and ^1 (quad perms) split the
+// rows by bits 2, 1, 0; under the xor order the value a lane keeps and the
+// one its partner sends sit in fixed registers, so no selects are needed.
+__device__ __forceinline__ float xor_rs8(const float (&x)[8]) {
+  float y[4], z[2];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) y[p] = x[p] + dpp<DPP_ROW_HMIRROR>(x[7 - p]);
+#pragma unroll
+  for (int p = 0; p < 2; ++p) z[p] = y[p] + dpp<DPP_QUAD_2301>(y[p ^ 2]);
+  return z[0] + dpp<DPP_QUAD_1032>(z[1]);
+}
+
+// x[p] = v of lane (lane xor p) within the lane's 8-lane block
+__device__ __forceinline__ void xor_gather8(float v, float (&x)[8]) {
+  x[0] = v;
+  x[1] = dpp<DPP_QUAD_1032>(v);
+  x[2] = dpp<DPP_QUAD_2301>(v);
+  x[3] = dpp<DPP_QUAD_3210>(v);
+  x[7] = dpp<DPP_ROW_HMIRROR>(v);
+  x[4] = dpp<DPP_QUAD_3210>(x[7]);
+  x[5] = dpp<DPP_QUAD_2301>(x[7]);
+  x[6] = dpp<DPP_QUAD_1032>(x[7]);
+}
+
 // min or max of x over the rows of part 0 (all lanes return it)
 template <bool MAX, int KC, int Q, typename T>
 __device__ __forceinline__ double hr_minmax(double x, int slot, HRShared<KC, Q, T> &sh) {
@@ -1207,7 +1287,7 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
 // beta^2 = w.w (exact squares summed in FP64, over the rows of part 0) and
 // the next matvec input lvn = w, published together behind one barrier
 template <int KC, int Q, typename T>
-__device__ __forceinline__ float hr_beta2(float w, HRShared<KC, Q, T> &sh) {
+__device__ __forceinline__ void hr_publish(float w, HRShared<KC, Q, T> &sh) {
   int t = threadIdx.x;
   asm volatile("" : "+v"(t));
   const int i = t & (KC - 1);
@@ -1217,10 +1297,30 @@ __device__ __forceinline__ float hr_beta2(float w, HRShared<KC, Q, T> &sh) {
     if ((t & 63) == 0) sh.rb[t >> 6] = sd;
   }
   __syncthreads();
+}
+// beta^2 published by hr_publish (valid until the next barrier that is
+// followed by a new publication)
+template <int KC, int Q, typename T>
+__device__ __forceinline__ float hr_b2(HRShared<KC, Q, T> &sh) {
   double tot = sh.rb[0];
 #pragma unroll
   for (int u = 1; u < HR<KC, Q>::WPH; ++u) tot += sh.rb[u];
   return (float)tot;
+}
+template <int KC, int Q, typename T>
+__device__ __forceinline__ float hr_beta2(float w, HRShared<KC, Q, T> &sh) {
+  hr_publish(w, sh);
+  return hr_b2(sh);
+}
+
+// beta = sqrt(beta^2) and 1 / beta from one reciprocal square root (1 ulp):
+// a short chain on the step's critical path, where sqrt and a correctly
+// rounded division took about 25 dependent instructions. beta^2 below the
+// normal range (where the instruction flushes its input and returns inf) or
+// NaN gives beta = 0, which the breakdown test always catches.
+__device__ __forceinline__ void hr_beta_scale(float b2, float &b, float &sc) {
+  sc = __builtin_amdgcn_rsqf(b2);
+  b = b2 >= 1.17549435e-38f ? b2 * sc : 0.f;
 }
 
 // true on every lane when p holds on any lane of the workgroup (a barrier,
@@ -1235,6 +1335,58 @@ __device__ __forceinline__ bool hr_any(bool p, HRShared<KC, Q, T> &sh) {
 #pragma unroll
   for (int u = 0; u < HR<KC, Q>::NW; ++u) any |= sh.ired[u];
   return any != 0;
+}
+
+// One column-dot unit: LDS columns 4 cg .. 4 cg + 3 over the 16-row blocks
+// [b0, b1) (b1 - b0 even), taken by the eight lanes g = 0..7 of a slot over
+// the row pairs 16 b + 2 g, 16 b + 2 g + 1; the unit's first lane writes the
+// four partials at cp[wo..wo+3].
+template <int KC, int Q, typename T>
+__device__ __forceinline__ void hr_dot_unit(const float *VL, HRShared<KC, Q, T> &sh, int g, int cg,
+                                            int b0, int b1, int wo) {
+  using C = HR<KC, Q>;
+  const float *col = VL + 2 * g * C::S + 4 * cg;
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (int b = b0; b < b1; b += 2) {
+    float4 v[4];
+    T l[4];
+#pragma unroll
+    for (int u2 = 0; u2 < 2; ++u2) {
+      const int r = 16 * (b + u2);
+      v[2 * u2] = *reinterpret_cast<const float4 *>(col + r * C::S);
+      v[2 * u2 + 1] = *reinterpret_cast<const float4 *>(col + (r + 1) * C::S);
+      // w read in FP32 and widened here (exact): half the LDS bytes of a T copy
+      const float2 x = *reinterpret_cast<const float2 *>(sh.lw + r + 2 * g);
+      l[2 * u2] = (T)x.x;
+      l[2 * u2 + 1] = (T)x.y;
+    }
+#pragma unroll
+    for (int u2 = 0; u2 < 4; ++u2) {
+      a0 = fmaT((T)v[u2].x, l[u2], a0);
+      a1 = fmaT((T)v[u2].y, l[u2], a1);
+      a2 = fmaT((T)v[u2].z, l[u2], a2);
+      a3 = fmaT((T)v[u2].w, l[u2], a3);
+    }
+  }
+  a0 += dpp<DPP_QUAD_1032>(a0);
+  a1 += dpp<DPP_QUAD_1032>(a1);
+  a2 += dpp<DPP_QUAD_1032>(a2);
+  a3 += dpp<DPP_QUAD_1032>(a3);
+  a0 += dpp<DPP_QUAD_2301>(a0);
+  a1 += dpp<DPP_QUAD_2301>(a1);
+  a2 += dpp<DPP_QUAD_2301>(a2);
+  a3 += dpp<DPP_QUAD_2301>(a3);
+  a0 += dpp<DPP_ROW_HMIRROR>(a0);
+  a1 += dpp<DPP_ROW_HMIRROR>(a1);
+  a2 += dpp<DPP_ROW_HMIRROR>(a2);
+  a3 += dpp<DPP_ROW_HMIRROR>(a3);
+  if (g == 0) {
+    T *cq = sh.cp + wo;
+    cq[0] = a0;
+    cq[1] = a1;
+    cq[2] = a2;
+    cq[3] = a3;
+  }
 }
 
 // One classical Gram-Schmidt pass of w (row i's entry, identical on all row
@@ -1256,8 +1408,35 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   if constexpr (WANT_PRE) ws = g_sum_dpp<64>(w * w);
   if (h == 0) {
     sh.lw[i] = w;
-    if constexpr (sizeof(T) == 8) sh.lwT[i] = (T)w;
     if (WANT_PRE && q == 0) sh.rp[wv] = ws;
+  }
+  // (0) KC = 256, register columns L + Q m + h: eight-row blocks, each slot
+  // summed over its eight rows in the lane and reduce-scattered over the
+  // blocks of the wavefront (slots past the part's formed columns skipped,
+  // unformed columns inside a slot hold zeros). The rows' w come from the
+  // block's lanes by DPP, so this runs before w is published.
+  if constexpr (C::OCT) {
+    const int hu = __builtin_amdgcn_readfirstlane(h);
+    const int nmh = j >= C::L + hu ? (j - C::L - hu) / Q + 1 : 0;
+    const int smax = (nmh + 7) >> 3;
+    if (smax > 0) {
+      float wf[8];
+      xor_gather8(w, wf);
+      T acc[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        T a = 0;
+        if (s < C::RB && s < smax) {
+#pragma unroll
+          for (int p = 0; p < 8; ++p)
+            a = fmaT((T)vr[8 * (s < C::RB ? s : 0) + p], (T)wf[p], a);
+        }
+        acc[s] = a;
+      }
+      const T r = blk_rs<T>(acc, q);
+      const int m = (t & 7) + 8 * ((q >> 3) & 7);
+      if (m < C::R) sh.rpart(h, wv)[m] = r;
+    }
   }
   __syncthreads();
   if constexpr (WANT_PRE) {
@@ -1273,21 +1452,21 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   // P is the largest power of two <= 8 with ncg P <= 64 (a part holds at
   // least two 16-row blocks), so a step with few columns spreads its dots
   // over every SIMD and both wavefronts of each; the part is uniform per
-  // wavefront and adjacent slots take adjacent groups. Past 32 groups
-  // wavefronts 0..3 take groups 0..31 whole and wavefronts 4..7 the rows of
-  // the remaining groups in four parts, beside their share of the register
-  // columns. At KC = 128 (three subsets per CU) P = 1. A unit's eight lanes
-  // take its four columns over the row pairs 16 b + 2 g, 16 b + 2 g + 1
-  // (g = t % 8) of its blocks: one 16-byte read of V per row feeds four
-  // FMAs, and one read of w serves both rows of a pair. With the row stride
-  // S (S / 4 odd), the even rows of the eight lanes fall in distinct even
-  // bank quads and the odd rows in odd ones, so the two units of a quarter
-  // wavefront read disjoint banks. The eight lanes' partial sums meet by
-  // DPP, and each unit's first lane writes its four partials, part-major
-  // (part p of column c at p (256 / P) + c; the tail parts at 128 + 32 p +
-  // c - 128), so the coefficient sums read each part with consecutive lanes
-  // on consecutive words and add the parts by a fixed tree. Rows >= k and
-  // columns >= ncl are zero.
+  // wavefront and adjacent slots take adjacent groups. Past 32 groups (KC =
+  // 256) every wavefront takes a half of the rows of eight of groups 0..31
+  // (P = 2) and then one eighth of the rows of the tail group 32 + slot, so
+  // the two wavefronts of each SIMD carry equal shares. At KC = 128 (three
+  // subsets per CU) P = 1. A unit's eight lanes take its four columns over
+  // the row pairs 16 b + 2 g, 16 b + 2 g + 1 (g = t % 8) of its blocks: one
+  // 16-byte read of V per row feeds four FMAs, and one read of w serves both
+  // rows of a pair. With the row stride S (S / 4 odd), the even rows of the
+  // eight lanes fall in distinct even bank quads and the odd rows in odd
+  // ones, so the two units of a quarter wavefront read disjoint banks. The
+  // eight lanes' partial sums meet by DPP, and each unit's first lane writes
+  // its four partials, part-major (part p of column c at p (256 / P) + c;
+  // the tail parts at 256 + 32 p + c - 128), so the coefficient sums read
+  // each part with consecutive lanes on consecutive words and add the parts
+  // by a fixed tree. Rows >= k and columns >= ncl are zero.
   const int ncl = min(j + 1, C::L);
   const int ncg = (ncl + 3) >> 2;
   constexpr int SL = 8 * C::NW, PMAX = KC == 256 ? 8 : 1;
@@ -1298,6 +1477,7 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   {
     static_assert((C::S / 4) % 2 == 1, "bank layout of the dot reads");
     static_assert(PMAX <= C::NW, "a wavefront takes one row part");
+    static_assert(KC != 256 || C::L <= 4 * 40, "tail groups fit the eight slots");
     const int wv0 = t >> 6, sl = (t >> 3) & 7, g = t & 7;
     // 16-row blocks rounded up to even (rows beyond k hold zeros, rows < KC
     // exist); a part's blocks [b0, b1), nbp = blocks per part, even
@@ -1310,75 +1490,31 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
       wo = pp * ((4 * SL) >> lp) + 4 * cg;
       b0 = pp * nbp;
       b1 = min(b0 + nbp, nbe);
-    } else if (wv0 < 4) {
-      cg = 8 * wv0 + sl;
-      wo = 4 * cg;
-      b0 = 0;
-      b1 = nbe;
     } else {
-      const int pp = wv0 - 4, nbp = (((nbe + 3) >> 2) + 1) & ~1;
-      cg = 32 + sl;
-      wo = 128 + 32 * pp + 4 * sl;
+      const int pp = wv0 & 1, nbp = (((nbe + 1) >> 1) + 1) & ~1;
+      cg = 8 * (wv0 >> 1) + sl;
+      wo = pp * 128 + 4 * cg;
       b0 = pp * nbp;
       b1 = min(b0 + nbp, nbe);
     }
     b0 = __builtin_amdgcn_readfirstlane(b0);
     b1 = __builtin_amdgcn_readfirstlane(b1);
-    if (cg < ncg) {
-      const float *col = VL + 2 * g * C::S + 4 * cg;
-      T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-      for (int b = b0; b < b1; b += 2) {
-        float4 v[4];
-        T l[4];
-#pragma unroll
-        for (int u2 = 0; u2 < 2; ++u2) {
-          const int r = 16 * (b + u2);
-          v[2 * u2] = *reinterpret_cast<const float4 *>(col + r * C::S);
-          v[2 * u2 + 1] = *reinterpret_cast<const float4 *>(col + (r + 1) * C::S);
-          if constexpr (sizeof(T) == 8) {
-            const double2 x = *reinterpret_cast<const double2 *>(sh.lwT + r + 2 * g);
-            l[2 * u2] = x.x;
-            l[2 * u2 + 1] = x.y;
-          } else {
-            const float2 x = *reinterpret_cast<const float2 *>(sh.lw + r + 2 * g);
-            l[2 * u2] = x.x;
-            l[2 * u2 + 1] = x.y;
-          }
-        }
-#pragma unroll
-        for (int u2 = 0; u2 < 4; ++u2) {
-          a0 = fmaT((T)v[u2].x, l[u2], a0);
-          a1 = fmaT((T)v[u2].y, l[u2], a1);
-          a2 = fmaT((T)v[u2].z, l[u2], a2);
-          a3 = fmaT((T)v[u2].w, l[u2], a3);
-        }
-      }
-      a0 += dpp<DPP_QUAD_1032>(a0);
-      a1 += dpp<DPP_QUAD_1032>(a1);
-      a2 += dpp<DPP_QUAD_1032>(a2);
-      a3 += dpp<DPP_QUAD_1032>(a3);
-      a0 += dpp<DPP_QUAD_2301>(a0);
-      a1 += dpp<DPP_QUAD_2301>(a1);
-      a2 += dpp<DPP_QUAD_2301>(a2);
-      a3 += dpp<DPP_QUAD_2301>(a3);
-      a0 += dpp<DPP_ROW_HMIRROR>(a0);
-      a1 += dpp<DPP_ROW_HMIRROR>(a1);
-      a2 += dpp<DPP_ROW_HMIRROR>(a2);
-      a3 += dpp<DPP_ROW_HMIRROR>(a3);
-      if (g == 0) {
-        T *cq = sh.cp + wo;
-        cq[0] = a0;
-        cq[1] = a1;
-        cq[2] = a2;
-        cq[3] = a3;
-      }
+    if (cg < ncg) hr_dot_unit<KC, Q, T>(VL, sh, g, cg, b0, b1, wo);
+    if (split_tail) {
+      // tail group 32 + slot, row part wv0 of eight
+      const int nbp = (((nbe + 7) >> 3) + 1) & ~1;
+      int tb0 = wv0 * nbp, tb1 = min(tb0 + nbp, nbe);
+      tb0 = __builtin_amdgcn_readfirstlane(tb0);
+      tb1 = __builtin_amdgcn_readfirstlane(tb1);
+      if (32 + sl < ncg)
+        hr_dot_unit<KC, Q, T>(VL, sh, g, 32 + sl, tb0, tb1, 256 + HRShared<KC, Q, T>::TST * wv0 + 4 * sl);
     }
   }
-  // (2) register columns L + Q m + h (m < nm): products reduce-scattered per
-  // chunk, the chunks holding formed columns only (uniform branch per chunk;
-  // unformed columns inside a chunk hold zeros)
+  // (2) register columns L + Q m + h (m < nm), KC = 128: products
+  // reduce-scattered per chunk, the chunks holding formed columns only
+  // (uniform branch per chunk; unformed columns inside a chunk hold zeros)
   int nm = 0;
-  if constexpr (C::R > 0) {
+  if constexpr (!C::OCT && C::R > 0) {
     nm = (max(0, j + 1 - C::L) + Q - 1) / Q;
     if (nm > 0) {
 #pragma unroll
@@ -1391,7 +1527,7 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
           x[u] = (m < C::R) ? (T)vr[m < C::R ? m : 0] * (T)w : (T)0;
         }
         const T r = wave_rs<NC, T>(x, q);
-        if ((q & ((1 << SH) - 1)) == 0) sh.rpart[h][wv][NC * c + (q >> SH)] = r;
+        if ((q & ((1 << SH) - 1)) == 0) sh.rpart(h, wv)[NC * c + (q >> SH)] = r;
       }
     }
   }
@@ -1403,10 +1539,17 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     T hc = 0;
     if (i < ncl) {
       // the column's parts, by a pairwise tree: P of them (a wave-uniform
-      // count) at i + p 4 SL / P, or the four tail parts at i + 32 p
-      if (split_tail && i >= 128) {
-        const T *cq = sh.cp + i;
-        hc = (cq[0] + cq[32]) + (cq[64] + cq[96]);
+      // count) at i + p 4 SL / P; past 32 groups two at i, i + 128, or the
+      // eight tail parts at 128 + i + 32 p
+      if (split_tail) {
+        if (i < 128) {
+          hc = sh.cp[i] + sh.cp[i + 128];
+        } else {
+          constexpr int TS = HRShared<KC, Q, T>::TST;
+          const T *cq = sh.cp + 128 + i;
+          hc = ((cq[0] + cq[TS]) + (cq[2 * TS] + cq[3 * TS])) +
+               ((cq[4 * TS] + cq[5 * TS]) + (cq[6 * TS] + cq[7 * TS]));
+        }
       } else if (lp == 0) {
         hc = sh.cp[i];
       } else if (lp == 1) {
@@ -1422,15 +1565,17 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
       }
     } else if constexpr (C::R > 0) {
       if (i >= C::L && i <= j) {
-        // column i is the m-th register column of part hh, in slot
-        // nm_hh - 1 - m with nm_hh = (j - L - hh) / Q + 1 formed in that part
+        // column i is the m-th register column of part hh: at KC = 256 its
+        // partials sit at m, its coefficient goes to [m mod 8][m / 8]; else
+        // in slot nm_hh - 1 - m with nm_hh = (j - L - hh) / Q + 1 formed in
+        // that part
         const int m = (i - C::L) / Q, hh = (i - C::L) % Q;
-        const int sl = (j - C::L - hh) / Q - m;
-        T s = sh.rpart[hh][0][sl];
+        const int sl = C::OCT ? m : (j - C::L - hh) / Q - m;
+        T s = sh.rpart(hh, 0)[sl];
 #pragma unroll
-        for (int u = 1; u < C::WPH; ++u) s += sh.rpart[hh][u][sl];
+        for (int u = 1; u < C::WPH; ++u) s += sh.rpart(hh, u)[sl];
         hc = s;
-        sh.lhr[hh][sl] = (float)hc;
+        sh.lhr[hh][C::OCT ? 8 * (m & 7) + (m >> 3) : sl] = (float)hc;
       }
     }
     sh.lh[i] = (float)hc;
@@ -1492,7 +1637,31 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
       u3 = __builtin_fmaf(ra.w, ha.w, u3);
     }
   }
-  if constexpr (C::R > 0) {
+  if constexpr (C::OCT) {
+    // the lane's slots (eight rows each) times their coefficients, read as
+    // two 16-byte words per lane (slots past the formed columns hold zeros,
+    // as do their coefficients); the block's lanes then sum each row's
+    // partials
+    const int hu = __builtin_amdgcn_readfirstlane(h);
+    const int nmh = j >= C::L + hu ? (j - C::L - hu) / Q + 1 : 0;
+    const int smax = (nmh + 7) >> 3;
+    if (smax > 0) {
+      const float4 *lr = reinterpret_cast<const float4 *>(sh.lhr[hu] + 8 * (t & 7));
+      const float4 c0 = lr[0], c1 = lr[1];
+      const float cf[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+      float x[8];
+#pragma unroll
+      for (int p = 0; p < 8; ++p) x[p] = 0.f;
+#pragma unroll
+      for (int s = 0; s < C::RB; ++s) {
+        if (s < smax) {
+#pragma unroll
+          for (int p = 0; p < 8; ++p) x[p] = __builtin_fmaf(vr[8 * s + p], cf[s], x[p]);
+        }
+      }
+      u0 += xor_rs8(x);
+    }
+  } else if constexpr (C::R > 0) {
     // slots 0..nm_h-1 of this part, coefficients read 4 at a time in slot
     // order (a broadcast: the part is uniform per wavefront), the four reads
     // of a block of 16 slots issued together (slots past nm_h hold zeros)
@@ -1689,61 +1858,91 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
         float pre = 0.f, unused;
         w = hr_cgs<KC, Q, T, true>(w, j, k, VL, vr, sh, pre);
         float b2 = hr_beta2(w, sh);
-        if (b2 < 0.5f * pre) {  // DGKS: w was mostly in span(V); orthogonalise again
-          w = hr_cgs<KC, Q, T, false>(w, j, k, VL, vr, sh, unused);
-          b2 = hr_beta2(w, sh);
-        }
-        const float b = sqrtf(b2);
-        vp = vi;
-        if (b > TAU * (amax + bmax)) {
-          bj = b;
-          bmax = fmaxf(bmax, b);
-          scale = 1.0f / b;
-        } else {
-          // restart: e_i for the first i of largest residual 1 - ||V[i][:]||^2,
-          // orthogonalised twice against the basis; beta_{j+1} = 0
-          st |= ST_BREAKDOWN;
-          bj = 0.f;
-          float rs = 0.f;
-          if (h == 0)
-            for (int l = 0; l < C::L; ++l)
-              rs = __builtin_fmaf(VL[i * C::S + l], VL[i * C::S + l], rs);
-          if constexpr (C::R > 0) {
-#pragma unroll
-            for (int m = 0; m < C::R; ++m) rs = __builtin_fmaf(vr[m], vr[m], rs);
+        float sc, b;
+        hr_beta_scale(b2, b, sc);
+        bool brk = false;
+        if (b2 < 0.5f * pre || !(b > TAU * (amax + bmax))) {  // uniform, rare
+          if (b2 < 0.5f * pre) {  // DGKS: w was mostly in span(V); orthogonalise again
+            w = hr_cgs<KC, Q, T, false>(w, j, k, VL, vr, sh, unused);
+            b2 = hr_beta2(w, sh);
+            hr_beta_scale(b2, b, sc);
           }
-          __syncthreads();
-          sh.pw[h][i] = rs;
-          __syncthreads();
-          float rsum = sh.pw[0][i];
+          if (!(b > TAU * (amax + bmax))) {
+            // restart: e_i for the first i of largest residual 1 - ||V[i][:]||^2,
+            // orthogonalised twice against the basis; beta_{j+1} = 0
+            brk = true;
+            st |= ST_BREAKDOWN;
+            float rs = 0.f;
+            if (h == 0)
+              for (int l = 0; l < C::L; ++l)
+                rs = __builtin_fmaf(VL[i * C::S + l], VL[i * C::S + l], rs);
+            if constexpr (C::OCT) {
+              // squares per row of the block, summed over the block's lanes
+              float x[8];
 #pragma unroll
-          for (int hh = 1; hh < Q; ++hh) rsum += sh.pw[hh][i];
-          const float res = 1.f - rsum;
-          const int wb = restart_lane<64>(res, i, k);
-          const float wvv = __shfl(i < k ? res : -1.f, wb & 63, 64);
-          if (t < KC && q == 0) {
-            sh.red[t >> 6] = wvv;
-            sh.ired[t >> 6] = wb;
-          }
-          __syncthreads();
-          int bi = sh.ired[0];
-          float bv = sh.red[0];
-          for (int w2 = 1; w2 < C::WPH; ++w2)
-            if (sh.red[w2] > bv) {
-              bv = sh.red[w2];
-              bi = sh.ired[w2];
+              for (int p = 0; p < 8; ++p) x[p] = 0.f;
+#pragma unroll
+              for (int m = 0; m < C::RA; ++m) x[m & 7] = __builtin_fmaf(vr[m], vr[m], x[m & 7]);
+              rs += xor_rs8(x);
+            } else if constexpr (C::R > 0) {
+#pragma unroll
+              for (int m = 0; m < C::R; ++m) rs = __builtin_fmaf(vr[m], vr[m], rs);
             }
-          w = (i == bi) ? 1.f : 0.f;
-          w = hr_cgs<KC, Q, T, false>(w, j, k, VL, vr, sh, unused);
-          w = hr_cgs<KC, Q, T, false>(w, j, k, VL, vr, sh, unused);
-          scale = 1.0f / sqrtf(hr_beta2(w, sh));
+            __syncthreads();
+            sh.pw[h][i] = rs;
+            __syncthreads();
+            float rsum = sh.pw[0][i];
+#pragma unroll
+            for (int hh = 1; hh < Q; ++hh) rsum += sh.pw[hh][i];
+            const float res = 1.f - rsum;
+            const int wb = restart_lane<64>(res, i, k);
+            const float wvv = __shfl(i < k ? res : -1.f, wb & 63, 64);
+            if (t < KC && q == 0) {
+              sh.red[t >> 6] = wvv;
+              sh.ired[t >> 6] = wb;
+            }
+            __syncthreads();
+            int bi = sh.ired[0];
+            float bv = sh.red[0];
+            for (int w2 = 1; w2 < C::WPH; ++w2)
+              if (sh.red[w2] > bv) {
+                bv = sh.red[w2];
+                bi = sh.ired[w2];
+              }
+            w = (i == bi) ? 1.f : 0.f;
+            w = hr_cgs<KC, Q, T, false>(w, j, k, VL, vr, sh, unused);
+            w = hr_cgs<KC, Q, T, false>(w, j, k, VL, vr, sh, unused);
+            sc = 1.0f / sqrtf(hr_beta2(w, sh));
+          }
         }
-        vi = w * scale;
-        // basis column j + 1: LDS, or slot 0 of the register shift of its
-        // part (the part is uniform per wavefront)
+        vp = vi;
+        bj = brk ? 0.f : b;
+        bmax = brk ? bmax : fmaxf(bmax, b);
+        scale = sc;
+        vi = w * sc;
+        // basis column j + 1: LDS, or at KC = 256 slot m / 8 of lane m mod 8
+        // of each 8-row block of its part (the m-th column there), or else
+        // slot 0 of the register shift of its part (the part is uniform per
+        // wavefront)
         const int c = j + 1;
         if (c < C::L && h == 0) VL[i * C::S + c] = vi;
-        if constexpr (C::R > 0) {
+        if constexpr (C::OCT) {
+          const int hu = __builtin_amdgcn_readfirstlane(h);
+          const int cc = c - C::L;
+          if (cc >= 0 && cc % Q == hu) {
+            const int m = cc / Q, sm = m >> 3;
+            float x[8];
+            xor_gather8(vi, x);
+            const bool mine = (t & 7) == (m & 7);
+#pragma unroll
+            for (int s = 0; s < C::RB; ++s) {
+              if (s == sm) {
+#pragma unroll
+                for (int p = 0; p < 8; ++p) vr[8 * s + p] = mine ? x[p] : vr[8 * s + p];
+              }
+            }
+          }
+        } else if constexpr (C::R > 0) {
           const int hu = __builtin_amdgcn_readfirstlane(h);
           const int cc = c - C::L;
           if (cc >= 0 && cc % Q == hu) {
