@@ -58,6 +58,29 @@ def test_k256_vs_oracle(gpu, k):
         np.testing.assert_allclose(de, np.maximum(hi - 1.0, 1.0 - lo), rtol=1e-6, atol=1e-7)
 
 
+@pytest.mark.parametrize("k", [148, 149, 150, 151, 152, 165, 166, 167, 182, 230, 245, 246, 247, 254])
+def test_512_lane_register_blocks_vs_oracle(gpu, k):
+    """128 < k <= 256 on the 512-lane kernel: basis columns 0..147 in LDS,
+    column 148 + 2 m + h in register slot m / 8 of lane m mod 8 of each
+    8-row block of part h. A column takes part in the Gram-Schmidt passes
+    of steps >= its index and the last step runs none, so k = 150 and 151
+    are the first sizes that use a register column of part 0 and of part 1,
+    k = 166 / 167 the first that use slot 1, k = 246 / 247 slot 6, and
+    k = 148, 149 have none; every Gram-Schmidt coefficient form (forced FP32,
+    forced FP64, the default) against the oracle."""
+    assert solver.tuning_get(k) == (512, 4)
+    n = 1024
+    G = solver.gram(solver.gen_phi("gauss", 500 + k, 4 * k, n))
+    idx = solver.gen_subsets(61 + k, 24, k, n)
+    olo, ohi, _, ost = oracle.solve_batch(G.cpu().numpy(), idx.cpu().numpy(), k=k, nthreads=8)
+    assert np.all(ost == 0)
+    for r in ("f32", "f64", None):
+        lo, hi, de, st = _np(solver.solve(G, idx, k=k, reorth=r))
+        assert np.all(st == 0), r
+        assert rel(lo, olo).max() <= RTOL and rel(hi, ohi).max() <= RTOL, r
+        np.testing.assert_allclose(de, np.maximum(hi - 1.0, 1.0 - lo), rtol=1e-6, atol=1e-7)
+
+
 @pytest.mark.parametrize("k", [66, 92, 93, 94, 111, 127, 128])
 def test_256_lane_register_columns_vs_oracle(gpu, k):
     """64 < k <= 128 on the 256-lane kernel: basis columns 0..91 in LDS, the
