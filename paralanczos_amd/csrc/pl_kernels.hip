@@ -1597,9 +1597,8 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     // 4; the part is uniform per wavefront, so these reads are broadcasts)
     const int cn4 = (cn + 3) & ~3;
     int c = 0;
-    // 16 columns per round trip (at KC = 128 with FP32 coefficients only:
-    // the FP64 form there has no registers to spare for them)
-    for (; (sizeof(T) == 4 || KC == 256) && c + 16 <= cn4; c += 16) {
+    // 16 columns per round trip
+    for (; c + 16 <= cn4; c += 16) {
       float4 ra[4], ha[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
