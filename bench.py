@@ -38,7 +38,7 @@ Reported beside it, in the same JSON line:
     HBM fetch rate the profiler's counters give for it (profiles/);
   * gram: G = Phi^T Phi of the serving frame by pl_gram (matrix cores) against
     one FP32 library GEMM of the full product (set-up, outside the timing);
-  * cpu_baseline: the FP64 oracle on the box's host cores over a bounded
+  * cpu_baseline: the FP64 oracle on the GPU host's cores over a bounded
     sample of the same subsets, and (c1) over the whole k = 8 configuration
     on one thread and on all cores;
   * c1, c2: the uniform-k configurations (k = 8 on a 256^2 G, k = 32 on a
@@ -496,7 +496,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the path runs "
-              "on a single GPU; DESIGN.md describes how it would shard.", file=sys.stderr)
+              "on a single GPU (subsets are independent: a sharded run would replicate G "
+              "and deal subsets out to the GPUs).", file=sys.stderr)
         return 2
 
     import torch

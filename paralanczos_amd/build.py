@@ -1,7 +1,7 @@
 """Build recipe for libparalanczos.so (hipcc, gfx950), in-tree.
 
     python -m paralanczos_amd.build            # build (always recompiles)
-    python -m paralanczos_amd.build --asm      # also dump gfx950 assembly
+    python -m paralanczos_amd.build --asm DIR  # also dump gfx950 assembly into DIR
 
 The library lands next to this file so that it travels with the source tree.
 """
@@ -44,7 +44,9 @@ def flags() -> list[str]:
     ]
 
 
-def build(verbose: bool = False, asm: bool = False) -> str:
+def build(verbose: bool = False, asm: str | None = None) -> str:
+    """Compile libparalanczos.so in place; with asm, also write the device
+    assembly of pl_kernels.hip into that directory."""
     t0 = time.time()
     tmp = LIB + ".tmp"
     cmd = [hipcc()] + flags() + ["-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
@@ -56,7 +58,7 @@ def build(verbose: bool = False, asm: bool = False) -> str:
         raise RuntimeError("hipcc failed building libparalanczos.so")
     os.replace(tmp, LIB)
     if asm:
-        out = os.path.join(HERE, "..", "_work", "asm")
+        out = asm
         os.makedirs(out, exist_ok=True)
         acmd = [hipcc()] + flags() + ["--cuda-device-only", "-S", "-o",
                                       os.path.join(out, "pl_kernels.s"),
@@ -68,4 +70,6 @@ def build(verbose: bool = False, asm: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True, asm="--asm" in sys.argv)
+    # python -m paralanczos_amd.build [--asm DIR]
+    a = sys.argv[sys.argv.index("--asm") + 1] if "--asm" in sys.argv else None
+    build(verbose=True, asm=a)

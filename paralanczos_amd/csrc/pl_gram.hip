@@ -152,7 +152,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_kernel(GramArgs a) {
   const int64_t s = a.s0 + blockIdx.y;
   if constexpr (IND) {
     // an index outside the frame flags the output and leaves it unwritten;
-    // the verdict goes through the first word of the (not yet staged) LDS
+    // the block-wide flag goes through the first word of the (not yet staged) LDS
     bool bad = false;
     for (int64_t c = t; c < a.n; c += GTHREADS) {
       const int32_t v = a.cols[s * a.n + c];
