@@ -1532,9 +1532,11 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     }
   }
   __syncthreads();
-  // (3) the coefficients, lane t < KC for column t (zero beyond j): LDS
-  // columns as the sum of their group's row parts in part order, register
-  // columns summed over the part's wavefronts in wavefront order
+  // (3) the coefficients: LDS column t on lane t < KC of part 0 (zero
+  // beyond j), the sum of its group's row parts in part order; register
+  // column L + r on lane r of part 1, summed over the owning part's
+  // wavefronts in wavefront order (the two kinds on different wavefronts,
+  // so neither waits behind the other's branch)
   if (h == 0) {
     T hc = 0;
     if (i < ncl) {
@@ -1563,22 +1565,21 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
         hc = ((cq[0] + cq[st]) + (cq[2 * st] + cq[3 * st])) +
              ((cq[4 * st] + cq[5 * st]) + (cq[6 * st] + cq[7 * st]));
       }
-    } else if constexpr (C::R > 0) {
-      if (i >= C::L && i <= j) {
-        // column i is the m-th register column of part hh: at KC = 256 its
-        // partials sit at m, its coefficient goes to [m mod 8][m / 8]; else
-        // in slot nm_hh - 1 - m with nm_hh = (j - L - hh) / Q + 1 formed in
-        // that part
-        const int m = (i - C::L) / Q, hh = (i - C::L) % Q;
-        const int sl = C::OCT ? m : (j - C::L - hh) / Q - m;
-        T s = sh.rpart(hh, 0)[sl];
-#pragma unroll
-        for (int u = 1; u < C::WPH; ++u) s += sh.rpart(hh, u)[sl];
-        hc = s;
-        sh.lhr[hh][C::OCT ? 8 * (m & 7) + (m >> 3) : sl] = (float)hc;
-      }
     }
     sh.lh[i] = (float)hc;
+  } else if constexpr (C::R > 0) {
+    if (h == 1 && i < KC - C::L && C::L + i <= j) {
+      // column L + i is the m-th register column of part hh: at KC = 256
+      // its partials sit at m, its coefficient goes to [m mod 8][m / 8];
+      // else in slot nm_hh - 1 - m with nm_hh = (j - L - hh) / Q + 1 formed
+      // in that part
+      const int m = i / Q, hh = i % Q;
+      const int sl = C::OCT ? m : (j - C::L - hh) / Q - m;
+      T s = sh.rpart(hh, 0)[sl];
+#pragma unroll
+      for (int u = 1; u < C::WPH; ++u) s += sh.rpart(hh, u)[sl];
+      sh.lhr[hh][C::OCT ? 8 * (m & 7) + (m >> 3) : sl] = (float)s;
+    }
   }
   __syncthreads();
   // (4) update partials (FP32): part h takes LDS columns [h cs, (h + 1) cs)
