@@ -1073,8 +1073,9 @@ struct HRShared {
     // written after the broadcast of w, when pw is dead)
     T rpart_u[C::OCT ? 1 : RPN];
   };
-  // KC = 256: the register-column partials are written before the barrier
-  // that publishes w, while other wavefronts may still read pw
+  // KC = 256: with FP64 coefficients the register-column partials are
+  // written before the barrier that publishes w, while other wavefronts may
+  // still read pw
   alignas(16) T rpart_s[C::OCT ? RPN : 1];
   union alignas(16) {
     float lvn[KC];      // next Lanczos vector, unnormalised: the matvec input
@@ -1414,30 +1415,38 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   // summed over its eight rows in the lane and reduce-scattered over the
   // blocks of the wavefront (slots past the part's formed columns skipped,
   // unformed columns inside a slot hold zeros). The rows' w come from the
-  // block's lanes by DPP, so this runs before w is published.
-  if constexpr (C::OCT) {
-    const int hu = __builtin_amdgcn_readfirstlane(h);
-    const int nmh = j >= C::L + hu ? (j - C::L - hu) / Q + 1 : 0;
-    const int smax = (nmh + 7) >> 3;
-    if (smax > 0) {
-      float wf[8];
-      xor_gather8(w, wf);
-      T acc[8];
+  // block's lanes by DPP, so they need no LDS. With FP64 coefficients they
+  // run before w is published; with FP32 coefficients next to the LDS-column
+  // dots, in the order that lets the two wavefronts of a SIMD (one of each
+  // part) overlap the two (measured: FP32 983 -> 971 us per subset and CU at
+  // k = 256, FP64 1,094 -> 1,108, so FP64 keeps the first placement).
+  constexpr bool IL = C::OCT && sizeof(T) == 4;
+  auto reg_dots = [&] {
+    if constexpr (C::OCT) {
+      const int hu = __builtin_amdgcn_readfirstlane(h);
+      const int nmh = j >= C::L + hu ? (j - C::L - hu) / Q + 1 : 0;
+      const int smax = (nmh + 7) >> 3;
+      if (smax > 0) {
+        float wf[8];
+        xor_gather8(w, wf);
+        T acc[8];
 #pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        T a = 0;
-        if (s < C::RB && s < smax) {
+        for (int s = 0; s < 8; ++s) {
+          T a = 0;
+          if (s < C::RB && s < smax) {
 #pragma unroll
-          for (int p = 0; p < 8; ++p)
-            a = fmaT((T)vr[8 * (s < C::RB ? s : 0) + p], (T)wf[p], a);
+            for (int p = 0; p < 8; ++p)
+              a = fmaT((T)vr[8 * (s < C::RB ? s : 0) + p], (T)wf[p], a);
+          }
+          acc[s] = a;
         }
-        acc[s] = a;
+        const T r = blk_rs<T>(acc, q);
+        const int m = (t & 7) + 8 * ((q >> 3) & 7);
+        if (m < C::R) sh.rpart(h, wv)[m] = r;
       }
-      const T r = blk_rs<T>(acc, q);
-      const int m = (t & 7) + 8 * ((q >> 3) & 7);
-      if (m < C::R) sh.rpart(h, wv)[m] = r;
     }
-  }
+  };
+  if constexpr (C::OCT && !IL) reg_dots();
   __syncthreads();
   if constexpr (WANT_PRE) {
     float s = sh.rp[0];
@@ -1474,7 +1483,7 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   int lp = 0;  // log2 P, wave-uniform
   while ((2 << lp) <= PMAX && ncg * (2 << lp) <= SL) ++lp;
   lp = __builtin_amdgcn_readfirstlane(lp);
-  {
+  auto lds_dots = [&] {
     static_assert((C::S / 4) % 2 == 1, "bank layout of the dot reads");
     static_assert(PMAX <= C::NW, "a wavefront takes one row part");
     static_assert(KC != 256 || C::L <= 4 * 40, "tail groups fit the eight slots");
@@ -1509,6 +1518,19 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
       if (32 + sl < ncg)
         hr_dot_unit<KC, Q, T>(VL, sh, g, 32 + sl, tb0, tb1, 256 + HRShared<KC, Q, T>::TST * wv0 + 4 * sl);
     }
+  };
+  // FP32 coefficients: part 0 (the first wavefront of each SIMD) starts
+  // with its LDS-column dots, part 1 with its register-column dots
+  if constexpr (IL) {
+    if (__builtin_amdgcn_readfirstlane(h) == 0) {
+      lds_dots();
+      reg_dots();
+    } else {
+      reg_dots();
+      lds_dots();
+    }
+  } else {
+    lds_dots();
   }
   // (2) register columns L + Q m + h (m < nm), KC = 128: products
   // reduce-scattered per chunk, the chunks holding formed columns only
