@@ -231,6 +231,37 @@ def test_multisection_agrees_with_bisection(gpu):
             assert rel(a, b).max() <= 2.0 ** -22, k
 
 
+@pytest.mark.parametrize("reorth", ["f32", "f64"])
+@pytest.mark.parametrize("k", [100, 200, 250])
+def test_breakdown_both_coefficient_forms(gpu, k, reorth):
+    """The restart path with each Gram-Schmidt coefficient form, including
+    steps whose basis has register columns (k > 148 on the 512-lane kernel):
+    the identity (the start vector is an eigenvector, so every step restarts)
+    and a subset with a duplicated index (singular G_S, lambda_min = 0),
+    against FP64 LAPACK."""
+    n = 512
+    G = torch.eye(n, device="cuda", dtype=torch.float32).contiguous()
+    idx = torch.arange(k, dtype=torch.int32, device="cuda")
+    lo, hi, de, st = _np(solver.solve(G, idx, k=k, reorth=reorth))
+    assert st[0] & 1 and (st[0] & ~1) == 0
+    assert abs(lo[0] - 1) < 1e-6 and abs(hi[0] - 1) < 1e-6 and abs(de[0]) < 1e-6
+    G = solver.gram(solver.gen_phi("gauss", 4, 2 * k, n))
+    Gh = G.cpu().numpy()
+    rows = []
+    for r in range(3):
+        s = oracle.floyd(11, r, n, k)
+        s[5 + r] = s[k - 3 - r]
+        rows.append(s)
+    flat = np.concatenate(rows).astype(np.int32)
+    lo, hi, de, st = _np(solver.solve(G, torch.from_numpy(flat).cuda(), k=k, reorth=reorth))
+    for j, s in enumerate(rows):
+        A, _ = oracle.gather(Gh, s)
+        w = np.linalg.eigvalsh(A)
+        assert (st[j] & ~1) == 0, (j, st[j])
+        assert abs(lo[j] - w[0]) <= RTOL * w[-1], (j, lo[j], w[0])
+        assert rel(hi[j], w[-1]) <= RTOL, (j, hi[j], w[-1])
+
+
 @pytest.mark.parametrize("form", ["tuned", "plain_wave"])
 @pytest.mark.parametrize("k", [100, 200])
 def test_edge_cases_large(gpu, k, form):
