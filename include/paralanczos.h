@@ -98,6 +98,8 @@ int64_t pl_overlap_forks(void);
  *                    form makes one launch per distinct size (and waits for
  *                    the bin counts on the host).
  * Outputs: lmin, lmax, delta (FP32) and status (PL_SUB_* bits) per subset.
+ * nsub = 0 is an empty batch: nothing is launched, and idx and the output
+ * pointers may then be NULL (so may those of pl_upper and pl_solve_alleig).
  * Subsets with status BADINDEX/BADK/NONFINITE get NaN eigenvalues.
  */
 pl_status pl_solve(const float *G, int64_t n, int64_t ld, const int32_t *idx,

@@ -159,7 +159,8 @@ void make_bins(int form, int kmax, BinPlan &bp) {
 
 pl_status check_solve_args(const float *G, int64_t n, int64_t ld, const int32_t *idx, int k,
                            int64_t nsub) {
-  if (!G || !idx) return fail(PL_ERR_ARG, "G and idx must be non-null");
+  // an empty batch reads and writes nothing: idx may then be null
+  if (!G || (!idx && nsub > 0)) return fail(PL_ERR_ARG, "G and idx must be non-null");
   if (n < 1 || n > 0x7fffffffLL) return fail(PL_ERR_ARG, "n = %lld out of range", (long long)n);
   if (ld < n) return fail(PL_ERR_ARG, "ld = %lld < n = %lld", (long long)ld, (long long)n);
   if (nsub < 0 || nsub > 0x7fffffffLL)
@@ -404,7 +405,7 @@ pl_status run_job(const SolveJob &j, void *ws, hipStream_t s, int stages = 3) {
 pl_status check_job(const SolveJob &j) {
   pl_status st = check_solve_args(j.G, j.n, j.ld, j.idx, j.k, j.nsub);
   if (st != PL_OK) return st;
-  if (!j.lmin || !j.lmax || !j.delta || !j.status)
+  if (j.nsub > 0 && (!j.lmin || !j.lmax || !j.delta || !j.status))
     return fail(PL_ERR_ARG, "output pointers must be non-null");
   if (j.form == -1) {
     if (pl::variant_capacity(j.explicit_v.lanes) < j.k)
@@ -561,7 +562,7 @@ pl_status pl_upper(const float *G, int64_t n, int64_t ld, const int32_t *idx,
                    uint32_t *status, void *stream) {
   pl_status st = check_solve_args(G, n, ld, idx, k, nsub);
   if (st != PL_OK) return st;
-  if (!upper || !status) return fail(PL_ERR_ARG, "output pointers must be non-null");
+  if (nsub > 0 && (!upper || !status)) return fail(PL_ERR_ARG, "output pointers must be non-null");
   if (iters < 1 || iters > 1000000)
     return fail(PL_ERR_ARG, "iters = %d outside [1, 1000000]", iters);
   if (nsub == 0) return PL_OK;
@@ -868,7 +869,7 @@ pl_status pl_solve_alleig(const float *G, int64_t n, int64_t ld, const int32_t *
                           int64_t nsub, double *eig, double *det, int32_t *err, void *stream) {
   pl_status st = check_solve_args(G, n, ld, idx, k, nsub);
   if (st != PL_OK) return st;
-  if (!eig || !det || !err) return fail(PL_ERR_ARG, "output pointers must be non-null");
+  if (nsub > 0 && (!eig || !det || !err)) return fail(PL_ERR_ARG, "output pointers must be non-null");
   if (nsub == 0) return PL_OK;
   hipStream_t s = (hipStream_t)stream;
   Workspace w;

@@ -11,7 +11,7 @@ import pytest
 import torch
 
 import oracle
-from paralanczos_amd import solver
+from paralanczos_amd import ParaLanczosError, solver
 
 pytestmark = pytest.mark.gpu
 
@@ -223,3 +223,22 @@ def test_mixed_k_csr(gpu):
                                           offsets=off.cpu().numpy(), nthreads=8)
     assert np.all(st == 0) and np.all(ost == 0)
     assert rel(lo, olo).max() <= RTOL and rel(hi, ohi).max() <= RTOL
+
+
+def test_empty_batches(gpu):
+    """No subsets: the solve entries return empty outputs without launching
+    (uniform and CSR, tuned and plain forms, a plan); the serving engine
+    rejects an empty request with an argument error."""
+    G = solver.gram(solver.gen_phi("gauss", 7, 64, 256))
+    none = torch.empty(0, dtype=torch.int32, device="cuda")
+    off0 = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for form in ("tuned", "plain"):
+        for kw in ({"k": 8}, {"k": 8, "offsets": off0}):
+            out = solver.solve(G, none, form=form, **kw)
+            assert all(t.numel() == 0 for t in out)
+    p = solver.Plan(G, none, k=8)
+    p.execute()
+    assert p.lmin.numel() == 0
+    srv = solver.Server(G, max_subsets=1024, max_indices=1 << 16)
+    with pytest.raises(ParaLanczosError):
+        srv.submit(none, k=8)
