@@ -197,3 +197,36 @@ def test_autotune_edges(mixed):
     finally:
         solver.tuning_reset()
     assert solver.tuning_get(24) == (32, 4) and solver.tuning_get(100) == (256, 4)
+
+
+def test_padded_leading_dimension(gpu):
+    """G stored with a leading dimension above n (rows padded with NaN that
+    no kernel may read) gives the same outputs bit for bit as the contiguous
+    G, through the C ABI, for sizes on every kernel family (lane groups, the
+    256- and 512-lane kernels) in the tuned and plain forms."""
+    import ctypes
+
+    from paralanczos_amd import _lib
+
+    n, pad = 1000, 13
+    G = solver.gram(solver.gen_phi("gauss", 0xD1, 320, n))
+    Gp = torch.full((n, n + pad), float("nan"), device="cuda")
+    Gp[:, :n] = G
+    ks = solver.gen_ks(0xD2, 400, [8, 16, 32, 64, 100, 128, 200, 256])
+    off = solver.offsets_from_ks(ks)
+    idx = solver.gen_subsets_csr(0xD3, off, n)
+    nsub = off.numel() - 1
+    L = _lib.load()
+    P = lambda x: ctypes.c_void_p(x.data_ptr())
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for form in ("tuned", "plain"):
+        ref = _np(solver.solve(G, idx, k=256, offsets=off, form=form))
+        out = [torch.empty(nsub, device="cuda") for _ in range(3)] + \
+              [torch.empty(nsub, dtype=torch.int32, device="cuda")]
+        rc = L.pl_solve(P(Gp), n, n + pad, P(idx), P(off), 256, nsub, *[P(x) for x in out],
+                        solver.FORMS[form], stream)
+        assert rc == 0
+        got = _np(out)
+        assert not got[3].any()
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), form
