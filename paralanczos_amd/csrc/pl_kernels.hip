@@ -1462,9 +1462,11 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   // least two 16-row blocks), so a step with few columns spreads its dots
   // over every SIMD and both wavefronts of each; the part is uniform per
   // wavefront and adjacent slots take adjacent groups. Past 32 groups (KC =
-  // 256) every wavefront takes a half of the rows of eight of groups 0..31
-  // (P = 2) and then one eighth of the rows of the tail group 32 + slot, so
-  // the two wavefronts of each SIMD carry equal shares. At KC = 128 (three
+  // 256) every wavefront takes a part of the rows of eight of groups 0..31
+  // (P = 2: halves, or with FP64 coefficients 3/4 for the first wavefront of
+  // a SIMD and 1/4 for the second on the same groups, since the SIMD issues
+  // the first one first) and then one eighth of the rows of the tail group
+  // 32 + slot. At KC = 128 (three
   // subsets per CU) P = 1. A unit's eight lanes take its four columns over
   // the row pairs 16 b + 2 g, 16 b + 2 g + 1 (g = t % 8) of its blocks: one
   // 16-byte read of V per row feeds four FMAs, and one read of w serves both
@@ -1500,11 +1502,25 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
       b0 = pp * nbp;
       b1 = min(b0 + nbp, nbe);
     } else {
-      const int pp = wv0 & 1, nbp = (((nbe + 1) >> 1) + 1) & ~1;
-      cg = 8 * (wv0 >> 1) + sl;
-      wo = pp * 128 + 4 * cg;
-      b0 = pp * nbp;
-      b1 = min(b0 + nbp, nbe);
+      if constexpr (sizeof(T) == 8) {
+        // FP64: the two wavefronts of a SIMD (w, w + 4) on the same eight
+        // groups, the first (which the SIMD issues first) over 3/4 of the
+        // blocks, the second over the rest (measured 1,087 -> 1,082 us per
+        // subset and CU at k = 256 against halves on wavefront pairs w,
+        // w + 1; 5/8 and 7/8 did worse, and FP32 keeps the halves)
+        const int pp = wv0 >> 2;
+        const int nb0 = min(nbe, ((nbe * 3 + 3) / 4 + 1) & ~1);
+        cg = 8 * (wv0 & 3) + sl;
+        wo = pp * 128 + 4 * cg;
+        b0 = pp ? nb0 : 0;
+        b1 = pp ? nbe : nb0;
+      } else {
+        const int pp = wv0 & 1, nbp = (((nbe + 1) >> 1) + 1) & ~1;
+        cg = 8 * (wv0 >> 1) + sl;
+        wo = pp * 128 + 4 * cg;
+        b0 = pp * nbp;
+        b1 = min(b0 + nbp, nbe);
+      }
     }
     b0 = __builtin_amdgcn_readfirstlane(b0);
     b1 = __builtin_amdgcn_readfirstlane(b1);
