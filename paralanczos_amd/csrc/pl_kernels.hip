@@ -1466,7 +1466,8 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   // (P = 2: halves, or with FP64 coefficients 3/4 for the first wavefront of
   // a SIMD and 1/4 for the second on the same groups, since the SIMD issues
   // the first one first) and then one eighth of the rows of the tail group
-  // 32 + slot. At KC = 128 (three
+  // 32 + slot (with FP64 coefficients one quarter, on the second wavefront
+  // of each SIMD). At KC = 128 (three
   // subsets per CU) P = 1. A unit's eight lanes take its four columns over
   // the row pairs 16 b + 2 g, 16 b + 2 g + 1 (g = t % 8) of its blocks: one
   // 16-byte read of V per row feeds four FMAs, and one read of w serves both
@@ -1475,7 +1476,7 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   // ones, so the two units of a quarter wavefront read disjoint banks. The
   // eight lanes' partial sums meet by DPP, and each unit's first lane writes
   // its four partials, part-major (part p of column c at p (256 / P) + c;
-  // the tail parts at 256 + 32 p + c - 128), so the coefficient sums read
+  // the tail parts at 256 + TST p + c - 128), so the coefficient sums read
   // each part with consecutive lanes on consecutive words and add the parts
   // by a fixed tree. Rows >= k and columns >= ncl are zero.
   const int ncl = min(j + 1, C::L);
@@ -1526,13 +1527,20 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     b1 = __builtin_amdgcn_readfirstlane(b1);
     if (cg < ncg) hr_dot_unit<KC, Q, T>(VL, sh, g, cg, b0, b1, wo);
     if (split_tail) {
-      // tail group 32 + slot, row part wv0 of eight
-      const int nbp = (((nbe + 7) >> 3) + 1) & ~1;
-      int tb0 = wv0 * nbp, tb1 = min(tb0 + nbp, nbe);
-      tb0 = __builtin_amdgcn_readfirstlane(tb0);
-      tb1 = __builtin_amdgcn_readfirstlane(tb1);
-      if (32 + sl < ncg)
-        hr_dot_unit<KC, Q, T>(VL, sh, g, 32 + sl, tb0, tb1, 256 + HRShared<KC, Q, T>::TST * wv0 + 4 * sl);
+      // tail group 32 + slot: row part wv0 of eight, or with FP64
+      // coefficients row part wv0 - 4 of four on the second wavefront of each
+      // SIMD only (it holds 1/4 of the main rows; measured 1,083 -> 1,079 us
+      // per subset and CU, 1,100 with the tail on the first wavefronts)
+      constexpr int TP = sizeof(T) == 8 ? 4 : 8;
+      const int tw = sizeof(T) == 8 ? wv0 - 4 : wv0;
+      if (tw >= 0 && tw < TP) {
+        const int nbp = (((nbe + TP - 1) / TP) + 1) & ~1;
+        int tb0 = tw * nbp, tb1 = min(tb0 + nbp, nbe);
+        tb0 = __builtin_amdgcn_readfirstlane(tb0);
+        tb1 = __builtin_amdgcn_readfirstlane(tb1);
+        if (32 + sl < ncg)
+          hr_dot_unit<KC, Q, T>(VL, sh, g, 32 + sl, tb0, tb1, 256 + HRShared<KC, Q, T>::TST * tw + 4 * sl);
+      }
     }
   };
   // FP32 coefficients: part 0 (the first wavefront of each SIMD) starts
@@ -1587,8 +1595,11 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
         } else {
           constexpr int TS = HRShared<KC, Q, T>::TST;
           const T *cq = sh.cp + 128 + i;
-          hc = ((cq[0] + cq[TS]) + (cq[2 * TS] + cq[3 * TS])) +
-               ((cq[4 * TS] + cq[5 * TS]) + (cq[6 * TS] + cq[7 * TS]));
+          if constexpr (sizeof(T) == 8)
+            hc = (cq[0] + cq[TS]) + (cq[2 * TS] + cq[3 * TS]);
+          else
+            hc = ((cq[0] + cq[TS]) + (cq[2 * TS] + cq[3 * TS])) +
+                 ((cq[4 * TS] + cq[5 * TS]) + (cq[6 * TS] + cq[7 * TS]));
         }
       } else if (lp == 0) {
         hc = sh.cp[i];
