@@ -1055,9 +1055,9 @@ struct HRShared {
                                      : 1;  // register partial slots per (part, wavefront)
   static constexpr int RPN = C::Q_ * C::WPH * RP;
   // LDS-column coefficient partials: four per dot unit (8 NW units); at
-  // KC = 256 also the tail groups' eight row parts (256 + TST p + c - 128)
+  // KC = 256 also the tail groups' four row parts (256 + TST p + c - 128)
   static constexpr int TST = (C::L - 128 + 3) / 4 * 4;  // tail columns, rounded to groups
-  static constexpr int CPN = KC == 256 ? 256 + 8 * TST : 32 * C::NW;
+  static constexpr int CPN = KC == 256 ? 256 + 4 * TST : 32 * C::NW;
   // every buffer read 16 bytes at a time starts on a 16-byte boundary
   union alignas(16) {
     float lw[KC];    // w broadcast
@@ -1461,13 +1461,13 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   // P is the largest power of two <= 8 with ncg P <= 64 (a part holds at
   // least two 16-row blocks), so a step with few columns spreads its dots
   // over every SIMD and both wavefronts of each; the part is uniform per
-  // wavefront and adjacent slots take adjacent groups. Past 32 groups (KC =
-  // 256) every wavefront takes a part of the rows of eight of groups 0..31
-  // (P = 2: halves, or with FP64 coefficients 3/4 for the first wavefront of
-  // a SIMD and 1/4 for the second on the same groups, since the SIMD issues
-  // the first one first) and then one eighth of the rows of the tail group
-  // 32 + slot (with FP64 coefficients one quarter, on the second wavefront
-  // of each SIMD). At KC = 128 (three
+  // wavefront and adjacent slots take adjacent groups. At KC = 256 with two
+  // row parts (16 < ncg <= 32) the two wavefronts of a SIMD (w, w + 4) take
+  // the same eight groups, the first over 5/8 of the rows and the second
+  // over the rest, since the SIMD issues the first one first; past 32 groups
+  // the first takes 3/4 of the rows of its eight of groups 0..31, and the
+  // second the rest plus one quarter of the rows of the tail group 32 +
+  // slot. At KC = 128 (three
   // subsets per CU) P = 1. A unit's eight lanes take its four columns over
   // the row pairs 16 b + 2 g, 16 b + 2 g + 1 (g = t % 8) of its blocks: one
   // 16-byte read of V per row feeds four FMAs, and one read of w serves both
@@ -1476,7 +1476,7 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   // ones, so the two units of a quarter wavefront read disjoint banks. The
   // eight lanes' partial sums meet by DPP, and each unit's first lane writes
   // its four partials, part-major (part p of column c at p (256 / P) + c;
-  // the tail parts at 256 + TST p + c - 128), so the coefficient sums read
+  // the four tail parts at 256 + TST p + c - 128), so the coefficient sums read
   // each part with consecutive lanes on consecutive words and add the parts
   // by a fixed tree. Rows >= k and columns >= ncl are zero.
   const int ncl = min(j + 1, C::L);
@@ -1495,7 +1495,7 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     // exist); a part's blocks [b0, b1), nbp = blocks per part, even
     const int nbe = (((k + 15) >> 4) + 1) & ~1;
     int cg, wo, b0, b1;  // group, partial offset of its column 4 cg, blocks
-    if (!split_tail && !(sizeof(T) == 8 && KC == 256 && lp == 1)) {
+    if (!split_tail && !(KC == 256 && lp == 1)) {  // P parts, NW / P wavefronts each
       const int pp = wv0 & ((1 << lp) - 1);
       const int nbp = (((nbe + (1 << lp) - 1) >> lp) + 1) & ~1;
       cg = 8 * (wv0 >> lp) + sl;
@@ -1503,9 +1503,10 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
       b0 = pp * nbp;
       b1 = min(b0 + nbp, nbe);
     } else if (!split_tail) {
-      // FP64, two row parts (16 < ncg <= 32): the two wavefronts of a SIMD on
-      // the same eight groups, the one issued first over 5/8 of the blocks
-      // (measured 1,077 -> 1,073 us per subset and CU at k = 256; 3/4: 1,083)
+      // KC = 256, two row parts (16 < ncg <= 32): the two wavefronts of a
+      // SIMD (w, w + 4) on the same eight groups, the one issued first over
+      // 5/8 of the blocks (measured with FP64 coefficients 1,077 -> 1,073 us
+      // per subset and CU at k = 256; 3/4: 1,083)
       const int pp = wv0 >> 2;
       const int nb0 = min(nbe, ((nbe * 5 + 8 - 1) / 8 + 1) & ~1);
       cg = 8 * (wv0 & 3) + sl;
@@ -1513,36 +1514,26 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
       b0 = pp ? nb0 : 0;
       b1 = pp ? nbe : nb0;
     } else {
-      if constexpr (sizeof(T) == 8) {
-        // FP64: the two wavefronts of a SIMD (w, w + 4) on the same eight
-        // groups, the first (which the SIMD issues first) over 3/4 of the
-        // blocks, the second over the rest (measured 1,087 -> 1,082 us per
-        // subset and CU at k = 256 against halves on wavefront pairs w,
-        // w + 1; 5/8 and 7/8 did worse, and FP32 keeps the halves)
-        const int pp = wv0 >> 2;
-        const int nb0 = min(nbe, ((nbe * 3 + 3) / 4 + 1) & ~1);
-        cg = 8 * (wv0 & 3) + sl;
-        wo = pp * 128 + 4 * cg;
-        b0 = pp ? nb0 : 0;
-        b1 = pp ? nbe : nb0;
-      } else {
-        const int pp = wv0 & 1, nbp = (((nbe + 1) >> 1) + 1) & ~1;
-        cg = 8 * (wv0 >> 1) + sl;
-        wo = pp * 128 + 4 * cg;
-        b0 = pp * nbp;
-        b1 = min(b0 + nbp, nbe);
-      }
+      // past 32 groups: the same pairing, the first wavefront over 3/4 of
+      // the blocks and the second over the rest plus the tail groups below
+      // (measured 1,087 -> 1,079 us per subset and CU at k = 256 with FP64
+      // coefficients, 972 -> 964 with FP32, against halves on the pairs w,
+      // w + 1 and the tail over all eight wavefronts)
+      const int pp = wv0 >> 2;
+      const int nb0 = min(nbe, ((nbe * 3 + 3) / 4 + 1) & ~1);
+      cg = 8 * (wv0 & 3) + sl;
+      wo = pp * 128 + 4 * cg;
+      b0 = pp ? nb0 : 0;
+      b1 = pp ? nbe : nb0;
     }
     b0 = __builtin_amdgcn_readfirstlane(b0);
     b1 = __builtin_amdgcn_readfirstlane(b1);
     if (cg < ncg) hr_dot_unit<KC, Q, T>(VL, sh, g, cg, b0, b1, wo);
     if (split_tail) {
-      // tail group 32 + slot: row part wv0 of eight, or with FP64
-      // coefficients row part wv0 - 4 of four on the second wavefront of each
-      // SIMD only (it holds 1/4 of the main rows; measured 1,083 -> 1,079 us
-      // per subset and CU, 1,100 with the tail on the first wavefronts)
-      constexpr int TP = sizeof(T) == 8 ? 4 : 8;
-      const int tw = sizeof(T) == 8 ? wv0 - 4 : wv0;
+      // tail group 32 + slot, row part wv0 - 4 of four, on the second
+      // wavefront of each SIMD (it holds 1/4 of the main rows)
+      constexpr int TP = 4;
+      const int tw = wv0 - 4;
       if (tw >= 0 && tw < TP) {
         const int nbp = (((nbe + TP - 1) / TP) + 1) & ~1;
         int tb0 = tw * nbp, tb1 = min(tb0 + nbp, nbe);
@@ -1598,18 +1589,14 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     if (i < ncl) {
       // the column's parts, by a pairwise tree: P of them (a wave-uniform
       // count) at i + p 4 SL / P; past 32 groups two at i, i + 128, or the
-      // eight tail parts at 128 + i + 32 p
+      // four tail parts at 128 + i + TST p
       if (split_tail) {
         if (i < 128) {
           hc = sh.cp[i] + sh.cp[i + 128];
         } else {
           constexpr int TS = HRShared<KC, Q, T>::TST;
           const T *cq = sh.cp + 128 + i;
-          if constexpr (sizeof(T) == 8)
-            hc = (cq[0] + cq[TS]) + (cq[2 * TS] + cq[3 * TS]);
-          else
-            hc = ((cq[0] + cq[TS]) + (cq[2 * TS] + cq[3 * TS])) +
-                 ((cq[4 * TS] + cq[5 * TS]) + (cq[6 * TS] + cq[7 * TS]));
+          hc = (cq[0] + cq[TS]) + (cq[2 * TS] + cq[3 * TS]);
         }
       } else if (lp == 0) {
         hc = sh.cp[i];
