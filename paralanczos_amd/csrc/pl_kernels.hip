@@ -1219,6 +1219,22 @@ __device__ __forceinline__ float xor_rs8(const float (&x)[8]) {
   return z[0] + dpp<DPP_QUAD_1032>(z[1]);
 }
 
+// Sum over the 16 lanes of a row of their x[p xor (lane mod 16) / 2]: lane
+// c holds in x[p] its partial for row p xor (c >> 1) of an 8-row block and
+// gets the total of row c >> 1 (lanes 2 r and 2 r + 1 both). Partners ^15
+// (row_mirror), ^7 (row_half_mirror) and ^3 (quad perm) split the rows by
+// bits 2, 1, 0 of (c >> 1), ^1 adds the two lanes of each row; under the xor
+// order the kept and the sent values sit in fixed registers (no selects).
+__device__ __forceinline__ float xor_rs16(const float (&x)[8]) {
+  float y[4], z[2];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) y[p] = x[p] + dpp<DPP_ROW_MIRROR>(x[p ^ 7]);
+#pragma unroll
+  for (int p = 0; p < 2; ++p) z[p] = y[p] + dpp<DPP_ROW_HMIRROR>(y[p ^ 3]);
+  const float r = z[0] + dpp<DPP_QUAD_3210>(z[1]);
+  return r + dpp<DPP_QUAD_1032>(r);
+}
+
 // x[p] = v of lane (lane xor p) within the lane's 8-lane block
 __device__ __forceinline__ void xor_gather8(float v, float (&x)[8]) {
   x[0] = v;
@@ -1245,6 +1261,47 @@ __device__ __forceinline__ double hr_minmax(double x, int slot, HRShared<KC, Q, 
 #pragma unroll
   for (int u = 1; u < WPH; ++u) y = MAX ? fmax(y, r[u]) : fmin(y, r[u]);
   return y;
+}
+
+// The matvec of a lane's G_S tile: rows 8 rb + (p xor f) (f = cb / 2) and
+// the CW = KC / 16 columns CW cb .. CW cb + CW - 1, held as g[CW p + u],
+// where u runs over the lane's columns in 4-column blocks rotated by
+// cb / (16 / NB) (NB = CW / 4 blocks): then the 16 lanes of a DPP row read
+// x = lvn by 16-byte reads from 16 distinct bank quads at every u, where
+// unrotated the lanes cb, cb + 4, .. hit the same banks (16 CW words apart).
+// Each row's CW products are summed in u order, the row partials of the 16
+// lanes of a DPP row (one 8-row block) meet by xor_rs16, and the lane returns
+// the sum of row 8 rb + f (lanes 2 f and 2 f + 1 both). pa: alpha's FP64 term
+// of that row, the exact product v_r (row sum) with v_r = lvn_r scale as the
+// row's owner forms it, on the even lane of the pair (0 on the odd one).
+template <int KC>
+__device__ __forceinline__ float hr_tile_matvec(const float (&g)[KC / 2], const float *lvn, int cb,
+                                                int rb, int f, float scale, double &pa) {
+  constexpr int CW = KC / 16, NB = CW / 4;
+  static_assert(CW % 4 == 0, "16-byte reads of the vector");
+  const int rot = cb / (16 / NB);
+  float x[CW];
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const float4 t4 = reinterpret_cast<const float4 *>(lvn + CW * cb)[(u + rot) % NB];
+    x[4 * u] = t4.x;
+    x[4 * u + 1] = t4.y;
+    x[4 * u + 2] = t4.z;
+    x[4 * u + 3] = t4.w;
+  }
+  float y[8];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    float acc = 0.f;
+#pragma unroll
+    for (int c = 0; c < CW; ++c) acc = __builtin_fmaf(g[CW * p + c], x[c], acc);
+    y[p] = acc;
+  }
+  const float ys = xor_rs16(y);
+  // alpha: the exact product of the row's v and its sum, one lane per row
+  const float v = lvn[8 * rb + f] * scale;
+  pa = (cb & 1) ? 0.0 : (double)v * (double)ys;
+  return ys;
 }
 
 // sum_c g[c] * L[c] over N entries, L a 16-byte aligned LDS vector read by
@@ -1780,8 +1837,12 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
   // KC - 1 - pr), so every lane has about KC (KC + 1) / (2 NT) independent
   // loads in flight and each word of G is requested once. The words go to
   // LDS in the basis area (free until the recurrence starts), rectangle row
-  // stride TS = KC + 3, and each lane then reads its half row from there:
-  // G_S[i][c] is triangle entry (min(i, c), max(i, c)). ----
+  // stride TS = KC + 3, and each lane then reads its tile from there (rows
+  // 8 rb + (p xor f), columns CW cb + c: lane t has cb = t mod 16, rb = t /
+  // 16, f = cb / 2, CW = KC / 16, the 4-column blocks of a lane's columns
+  // rotated by cb / (16 / NB), NB = CW / 4, as hr_tile_matvec reads them; a
+  // DPP row of 16 lanes holds one 8-row block): G_S[r][c] is triangle entry
+  // (min(r, c), max(r, c)). ----
   float g[C::HW];
   bool fin = true;
   {
@@ -1811,29 +1872,43 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
       }
     }
     __syncthreads();
+    constexpr int CW = KC / 16, NB = CW / 4;
+    static_assert(8 * CW == C::HW && C::NT == 16 * (KC / 8), "one 8-row block per DPP row");
+    const int cb = t & 15, rb = t >> 4, f = cb >> 1, rot = cb / (16 / NB);
 #pragma unroll
-    for (int v = 0; v < C::HW; ++v) {
-      const int c = C::HW * h + v;
-      float y = 0.f;
-      if (st == 0 && i < k && c < k) {
-        const int r0 = min(i, c), c0 = max(i, c);
-        const bool first = r0 < KC / 2;
-        const int pr = first ? r0 : KC - 1 - r0;
-        const int qc = first ? c0 - r0 : c0 + 1;
-        y = VL[pr * TS + qc];
+    for (int p = 0; p < 8; ++p) {
+#pragma unroll
+      for (int u = 0; u < CW; ++u) {
+        const int r = 8 * rb + (p ^ f), c = CW * cb + 4 * (((u >> 2) + rot) % NB) + (u & 3);
+        float y = 0.f;
+        if (st == 0 && r < k && c < k) {
+          const int r0 = min(r, c), c0 = max(r, c);
+          const bool first = r0 < KC / 2;
+          const int pr = first ? r0 : KC - 1 - r0;
+          const int qc = first ? c0 - r0 : c0 + 1;
+          y = VL[pr * TS + qc];
+        }
+        g[CW * p + u] = y;
+        fin = fin && isfinite(y);
       }
-      g[v] = y;
-      fin = fin && isfinite(y);
     }
   }
   if (hr_any(!fin, sh)) st |= ST_NONFINITE;
 
   if constexpr (MODE == MODE_DUMP) {
-    if (i < k && st == 0) {
-      float *o = a.gs_out + s * (int64_t)k * k + (int64_t)i * k + C::HW * h;
+    if (st == 0) {
+      constexpr int CW = KC / 16, NB = CW / 4;
+      const int cb = t & 15, rb = t >> 4, f = cb >> 1, rot = cb / (16 / NB);
+      float *o = a.gs_out + s * (int64_t)k * k;
 #pragma unroll
-      for (int c = 0; c < C::HW; ++c)
-        if (C::HW * h + c < k) o[c] = g[c];
+      for (int p = 0; p < 8; ++p) {
+        const int r = 8 * rb + (p ^ f);
+#pragma unroll
+        for (int u = 0; u < CW; ++u) {
+          const int c = CW * cb + 4 * (((u >> 2) + rot) % NB) + (u & 3);
+          if (r < k && c < k) o[(int64_t)r * k + c] = g[CW * p + u];
+        }
+      }
     }
     return;
   } else if constexpr (MODE == MODE_CHECKSUM) {
@@ -1881,18 +1956,17 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
         int tl = threadIdx.x;
         asm volatile("" : "+v"(tl));
         const int il = tl & (KC - 1), hl = tl >> C::LOGK;
-        // matvec partial over this part's columns; alpha's partial over it:
-        // exact products v_i mv_i summed in FP64 (the parts' partials sum to
-        // v . G lvn)
-        const float mv = hr_rowdot<C::HW>(g, sh.lvn + C::HW * hl);
-        const double pa = wave_sum_f64((double)vi * (double)mv);
-        sh.pw[hl][il] = mv;
+        // matvec of the lane's tile, row sums through LDS; alpha's terms
+        // (exact products v_r times the row sum, one lane per row, summed in
+        // FP64 over the wavefront and then the wavefronts) exchanged alongside
+        const int cbl = tl & 15, rbl = tl >> 4, fl = cbl >> 1;
+        double pl;
+        const float mv = hr_tile_matvec<KC>(g, sh.lvn, cbl, rbl, fl, scale, pl);
+        const double pa = wave_sum_f64(pl);
+        if ((cbl & 1) == 0) sh.pw[0][8 * rbl + fl] = mv;
         if ((tl & 63) == 0) sh.ra[tl >> 6] = pa;
         __syncthreads();
-        float w = sh.pw[0][il];
-#pragma unroll
-        for (int hh = 1; hh < Q; ++hh) w += sh.pw[hh][il];
-        w *= scale;
+        float w = sh.pw[0][il] * scale;
         double asum = sh.ra[0];
 #pragma unroll
         for (int w2 = 1; w2 < C::NW; ++w2) asum += sh.ra[w2];
