@@ -1083,6 +1083,12 @@ struct HRShared {
     double redd[16];    // bounds reduction, after the recurrence
   };
   __device__ T *rpart(int h, int wv) { return (C::OCT ? rpart_s : rpart_u) + (h * C::WPH + wv) * RP; }
+  // FP64 coefficients: w widened once to FP64 for the column dots, so the
+  // dot loops convert only V's entries (at KC = 256 this takes 2 KiB of the
+  // 2.2 KiB of LDS left beside the basis; at KC = 128, 1 KiB per workgroup
+  // keeps three workgroups per CU)
+  static constexpr bool WD = sizeof(T) == 8;
+  alignas(16) double lwd[WD ? KC : 2];
   double ra[C::NW];     // alpha partials (FP64) of all wavefronts
   double rb[C::WPH];    // beta^2 partials (FP64) of the wavefronts of part 0
   float rp[C::WPH];     // ||w||^2 partials before a Gram-Schmidt pass
@@ -1413,10 +1419,16 @@ __device__ __forceinline__ void hr_dot_unit(const float *VL, HRShared<KC, Q, T> 
       const int r = 16 * (b + u2);
       v[2 * u2] = *reinterpret_cast<const float4 *>(col + r * C::S);
       v[2 * u2 + 1] = *reinterpret_cast<const float4 *>(col + (r + 1) * C::S);
-      // w read in FP32 and widened here (exact): half the LDS bytes of a T copy
-      const float2 x = *reinterpret_cast<const float2 *>(sh.lw + r + 2 * g);
-      l[2 * u2] = (T)x.x;
-      l[2 * u2 + 1] = (T)x.y;
+      if constexpr (HRShared<KC, Q, T>::WD) {
+        const double2 x = *reinterpret_cast<const double2 *>(sh.lwd + r + 2 * g);
+        l[2 * u2] = x.x;
+        l[2 * u2 + 1] = x.y;
+      } else {
+        // FP32 coefficients: w read in FP32
+        const float2 x = *reinterpret_cast<const float2 *>(sh.lw + r + 2 * g);
+        l[2 * u2] = (T)x.x;
+        l[2 * u2 + 1] = (T)x.y;
+      }
     }
 #pragma unroll
     for (int u2 = 0; u2 < 4; ++u2) {
@@ -1466,6 +1478,7 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   if constexpr (WANT_PRE) ws = g_sum_dpp<64>(w * w);
   if (h == 0) {
     sh.lw[i] = w;
+    if constexpr (HRShared<KC, Q, T>::WD) sh.lwd[i] = (double)w;
     if (WANT_PRE && q == 0) sh.rp[wv] = ws;
   }
   // (0) KC = 256, register columns L + Q m + h: eight-row blocks, each slot
