@@ -1438,24 +1438,41 @@ __device__ __forceinline__ void hr_dot_unit(const float *VL, HRShared<KC, Q, T> 
       a3 = fmaT((T)v[u2].w, l[u2], a3);
     }
   }
-  a0 += dpp<DPP_QUAD_1032>(a0);
-  a1 += dpp<DPP_QUAD_1032>(a1);
-  a2 += dpp<DPP_QUAD_1032>(a2);
-  a3 += dpp<DPP_QUAD_1032>(a3);
-  a0 += dpp<DPP_QUAD_2301>(a0);
-  a1 += dpp<DPP_QUAD_2301>(a1);
-  a2 += dpp<DPP_QUAD_2301>(a2);
-  a3 += dpp<DPP_QUAD_2301>(a3);
-  a0 += dpp<DPP_ROW_HMIRROR>(a0);
-  a1 += dpp<DPP_ROW_HMIRROR>(a1);
-  a2 += dpp<DPP_ROW_HMIRROR>(a2);
-  a3 += dpp<DPP_ROW_HMIRROR>(a3);
-  if (g == 0) {
-    T *cq = sh.cp + wo;
-    cq[0] = a0;
-    cq[1] = a1;
-    cq[2] = a2;
-    cq[3] = a3;
+  if constexpr (sizeof(T) == 8) {
+    // reduce-scatter over the eight lanes (a 64-bit DPP move is two moves,
+    // so halving the values each level beats summing all four everywhere):
+    // partner 7 - g splits the columns {0, 1} / {2, 3}, partner g xor 2
+    // splits each pair, partner g xor 1 completes; lanes 2c and 2c + 1 end
+    // with column c and the even one writes it
+    const bool lo = g < 4, pb = (g & 2) == 0;
+    T k0 = lo ? a0 : a2, k1 = lo ? a1 : a3;
+    const T s0 = lo ? a2 : a0, s1 = lo ? a3 : a1;
+    k0 += dpp<DPP_ROW_HMIRROR>(s0);
+    k1 += dpp<DPP_ROW_HMIRROR>(s1);
+    T kk = pb ? k0 : k1;
+    kk += dpp<DPP_QUAD_2301>(pb ? k1 : k0);
+    kk += dpp<DPP_QUAD_1032>(kk);
+    if ((g & 1) == 0) sh.cp[wo + (g >> 1)] = kk;
+  } else {
+    a0 += dpp<DPP_QUAD_1032>(a0);
+    a1 += dpp<DPP_QUAD_1032>(a1);
+    a2 += dpp<DPP_QUAD_1032>(a2);
+    a3 += dpp<DPP_QUAD_1032>(a3);
+    a0 += dpp<DPP_QUAD_2301>(a0);
+    a1 += dpp<DPP_QUAD_2301>(a1);
+    a2 += dpp<DPP_QUAD_2301>(a2);
+    a3 += dpp<DPP_QUAD_2301>(a3);
+    a0 += dpp<DPP_ROW_HMIRROR>(a0);
+    a1 += dpp<DPP_ROW_HMIRROR>(a1);
+    a2 += dpp<DPP_ROW_HMIRROR>(a2);
+    a3 += dpp<DPP_ROW_HMIRROR>(a3);
+    if (g == 0) {
+      T *cq = sh.cp + wo;
+      cq[0] = a0;
+      cq[1] = a1;
+      cq[2] = a2;
+      cq[3] = a3;
+    }
   }
 }
 
