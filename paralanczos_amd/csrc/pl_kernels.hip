@@ -1279,7 +1279,8 @@ __device__ __forceinline__ double hr_minmax(double x, int slot, HRShared<KC, Q, 
 // lanes of a DPP row (one 8-row block) meet by xor_rs16, and the lane returns
 // the sum of row 8 rb + f (lanes 2 f and 2 f + 1 both). pa: alpha's FP64 term
 // of that row, the exact product v_r (row sum) with v_r = lvn_r scale as the
-// row's owner forms it, on the even lane of the pair (0 on the odd one).
+// row's owner forms it, on both lanes of the pair (wave_sum_pairs_f64 counts
+// each row once; computing it on both lanes avoids a divergent branch).
 template <int KC>
 __device__ __forceinline__ float hr_tile_matvec(const float (&g)[KC / 2], const float *lvn, int cb,
                                                 int rb, int f, float scale, double &pa) {
@@ -1304,9 +1305,10 @@ __device__ __forceinline__ float hr_tile_matvec(const float (&g)[KC / 2], const 
     y[p] = acc;
   }
   const float ys = xor_rs16(y);
-  // alpha: the exact product of the row's v and its sum, one lane per row
+  // alpha: the exact product of the row's v and its sum, on both lanes of
+  // the row's pair (the caller sums over the even lanes only)
   const float v = lvn[8 * rb + f] * scale;
-  pa = (cb & 1) ? 0.0 : (double)v * (double)ys;
+  pa = (double)v * (double)ys;
   return ys;
 }
 
@@ -1338,14 +1340,19 @@ __device__ __forceinline__ float hr_rowdot(const float (&g)[N], const float *L) 
 // exact products of FP32 operands, it gives the extended-precision dot
 // product that compensated FP32 summation (Dot2) emulates, in fewer
 // instructions: FP64 is native here.
-__device__ __forceinline__ double wave_sum_f64(double x) {
-  x += dpp<DPP_QUAD_1032>(x);
+// the same sum when lanes 2l and 2l + 1 hold the same value: the total over
+// the even lanes, on every lane
+__device__ __forceinline__ double wave_sum_pairs_f64(double x) {
   x += dpp<DPP_QUAD_2301>(x);
   x += dpp<DPP_ROW_HMIRROR>(x);
   x += dpp<DPP_ROW_MIRROR>(x);
   x = swap_add<double>(x, x, false);
   x = swap_add<double>(x, x, true);
   return x;
+}
+__device__ __forceinline__ double wave_sum_f64(double x) {
+  x += dpp<DPP_QUAD_1032>(x);
+  return wave_sum_pairs_f64(x);
 }
 
 // beta^2 = w.w (exact squares summed in FP64, over the rows of part 0) and
@@ -1992,7 +1999,7 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
         const int cbl = tl & 15, rbl = tl >> 4, fl = cbl >> 1;
         double pl;
         const float mv = hr_tile_matvec<KC>(g, sh.lvn, cbl, rbl, fl, scale, pl);
-        const double pa = wave_sum_f64(pl);
+        const double pa = wave_sum_pairs_f64(pl);
         if ((cbl & 1) == 0) sh.pw[0][8 * rbl + fl] = mv;
         if ((tl & 63) == 0) sh.ra[tl >> 6] = pa;
         __syncthreads();
