@@ -1994,8 +1994,8 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
         asm volatile("" : "+v"(tl));
         const int il = tl & (KC - 1), hl = tl >> C::LOGK;
         // matvec of the lane's tile, row sums through LDS; alpha's terms
-        // (exact products v_r times the row sum, one lane per row, summed in
-        // FP64 over the wavefront and then the wavefronts) exchanged alongside
+        // (exact products v_r times the row sum, once per row, summed in FP64
+        // over the wavefront and then the wavefronts) exchanged alongside
         const int cbl = tl & 15, rbl = tl >> 4, fl = cbl >> 1;
         double pl;
         const float mv = hr_tile_matvec<KC>(g, sh.lvn, cbl, rbl, fl, scale, pl);
