@@ -1608,13 +1608,14 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
       b0 = pp ? nb0 : 0;
       b1 = pp ? nbe : nb0;
     } else {
-      // past 32 groups: the same pairing, the first wavefront over 3/4 of
+      // past 32 groups: the same pairing, the first wavefront over 7/8 of
       // the blocks and the second over the rest plus the tail groups below
-      // (measured 1,087 -> 1,079 us per subset and CU at k = 256 with FP64
-      // coefficients, 972 -> 964 with FP32, against halves on the pairs w,
-      // w + 1 and the tail over all eight wavefronts)
+      // (3/4 measured 1,087 -> 1,079 us per subset and CU at k = 256 with
+      // FP64 coefficients, 972 -> 964 with FP32, against halves on the pairs
+      // w, w + 1 and the tail over all eight wavefronts; since the FP64 dot
+      // changes 7/8 beats 3/4 by 0.4% in both forms, 5/8 and 8/8 lose)
       const int pp = wv0 >> 2;
-      const int nb0 = min(nbe, ((nbe * 3 + 3) / 4 + 1) & ~1);
+      const int nb0 = min(nbe, ((nbe * 13 + 15) / 16 + 1) & ~1);
       cg = 8 * (wv0 & 3) + sl;
       wo = pp * 128 + 4 * cg;
       b0 = pp ? nb0 : 0;
@@ -1625,7 +1626,7 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
     if (cg < ncg) hr_dot_unit<KC, Q, T>(VL, sh, g, cg, b0, b1, wo);
     if (split_tail) {
       // tail group 32 + slot, row part wv0 - 4 of four, on the second
-      // wavefront of each SIMD (it holds 1/4 of the main rows)
+      // wavefront of each SIMD (it holds 1/8 of the main rows)
       constexpr int TP = 4;
       const int tw = wv0 - 4;
       if (tw >= 0 && tw < TP) {
