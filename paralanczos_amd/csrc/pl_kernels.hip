@@ -1355,6 +1355,37 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
   return wave_sum_pairs_f64(x);
 }
 
+// The total over the wavefront on lane 63 only (other lanes: unspecified),
+// for x equal on the lanes of each row of 16 (a row total): rows 1 and 3 add
+// lane 15 of the row before (row_bcast:15), then rows 2 and 3 add lane 31
+// (row_bcast:31): two 64-bit DPP moves and an add per step, against two
+// copies, two permlane swaps and an add for the all-lane sum.
+// rows outside ROWS receive an unspecified value (only lane 63 is used)
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_rows_f64(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, ROWS, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, ROWS, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double rows_to_lane63_f64(double x) {
+  x += dpp_rows_f64<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_rows_f64<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+  return x;
+}
+// wave_sum_pairs_f64's total on lane 63
+__device__ __forceinline__ double wave_total63_pairs_f64(double x) {
+  x += dpp<DPP_QUAD_2301>(x);
+  x += dpp<DPP_ROW_HMIRROR>(x);
+  x += dpp<DPP_ROW_MIRROR>(x);
+  return rows_to_lane63_f64(x);
+}
+// wave_sum_f64's total on lane 63
+__device__ __forceinline__ double wave_total63_f64(double x) {
+  x += dpp<DPP_QUAD_1032>(x);
+  return wave_total63_pairs_f64(x);
+}
+
 // beta^2 = w.w (exact squares summed in FP64, over the rows of part 0) and
 // the next matvec input lvn = w, published together behind one barrier
 template <int KC, int Q, typename T>
@@ -1362,10 +1393,10 @@ __device__ __forceinline__ void hr_publish(float w, HRShared<KC, Q, T> &sh) {
   int t = threadIdx.x;
   asm volatile("" : "+v"(t));
   const int i = t & (KC - 1);
-  const double sd = wave_sum_f64((double)w * (double)w);
+  const double sd = wave_total63_f64((double)w * (double)w);
   if (t < KC) {
     sh.lvn[i] = w;
-    if ((t & 63) == 0) sh.rb[t >> 6] = sd;
+    if ((t & 63) == 63) sh.rb[t >> 6] = sd;
   }
   __syncthreads();
 }
@@ -2000,9 +2031,9 @@ __device__ __forceinline__ void hr_subset(const SolveArgs &a, float *VL, HRShare
         const int cbl = tl & 15, rbl = tl >> 4, fl = cbl >> 1;
         double pl;
         const float mv = hr_tile_matvec<KC>(g, sh.lvn, cbl, rbl, fl, scale, pl);
-        const double pa = wave_sum_pairs_f64(pl);
+        const double pa = wave_total63_pairs_f64(pl);
         if ((cbl & 1) == 0) sh.pw[0][8 * rbl + fl] = mv;
-        if ((tl & 63) == 0) sh.ra[tl >> 6] = pa;
+        if ((tl & 63) == 63) sh.ra[tl >> 6] = pa;
         __syncthreads();
         float w = sh.pw[0][il] * scale;
         double asum = sh.ra[0];
