@@ -1373,6 +1373,16 @@ __device__ __forceinline__ double rows_to_lane63_f64(double x) {
   x += dpp_rows_f64<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
   return x;
 }
+// g_sum_dpp<64>'s total on lane 63 only (other lanes: unspecified)
+__device__ __forceinline__ float wave_total63_f32(float x) {
+  x += dpp<DPP_QUAD_1032>(x);
+  x += dpp<DPP_QUAD_2301>(x);
+  x += dpp<DPP_ROW_HMIRROR>(x);
+  x += dpp<DPP_ROW_MIRROR>(x);
+  x += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x142, 0xa, 0xf, false));
+  x += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x143, 0xc, 0xf, false));
+  return x;
+}
 // wave_sum_pairs_f64's total on lane 63
 __device__ __forceinline__ double wave_total63_pairs_f64(double x) {
   x += dpp<DPP_QUAD_2301>(x);
@@ -1530,11 +1540,11 @@ __device__ __forceinline__ float hr_cgs(float w, int j, int k, const float *VL,
   asm volatile("" : "+v"(t));
   const int i = t & (KC - 1), h = t >> C::LOGK, wv = (t >> 6) & (C::WPH - 1), q = t & 63;
   float ws = 0.f;
-  if constexpr (WANT_PRE) ws = g_sum_dpp<64>(w * w);
+  if constexpr (WANT_PRE) ws = wave_total63_f32(w * w);  // on lane 63
   if (h == 0) {
     sh.lw[i] = w;
     if constexpr (HRShared<KC, Q, T>::WD) sh.lwd[i] = (double)w;
-    if (WANT_PRE && q == 0) sh.rp[wv] = ws;
+    if (WANT_PRE && q == 63) sh.rp[wv] = ws;
   }
   // (0) KC = 256, register columns L + Q m + h: eight-row blocks, each slot
   // summed over its eight rows in the lane and reduce-scattered over the
